@@ -1,0 +1,200 @@
+"""Benchmark: RK4 ray-steps/s and 1080p frame time of the HIP ray-cast pass on one MI355X.
+
+One step = one 1920x1080 frame for each of the four fields (swirl, lens, hot-air, warp): the
+ray-cast kernel (RK4 through the field, per-step segment queries against the Cornell box plus
+a 20,480-face displaced icosphere) followed by the Lyapunov overlay, exactly the pass linon
+dispatches in MainView::render (src/main_view.rs:831-849).  Inputs are resident on the
+device before the timed region; nothing is read back inside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line.  `roofline` is for the dominant kernel (the ray-cast instantiation with
+the most device time): FP32 FLOPs counted with the convention below from the kernel's own
+step / triangle-test counters, divided by its mean launch time from HIP events recorded on
+the launch stream over the timed region.  `cpu_baseline` times the FP32 CPU restatement
+(brute force over all faces, as the reference shader does) on a strided pixel sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import linon_amd as L  # noqa: E402
+from linon_amd import configs  # noqa: E402
+
+# FP32 FLOP convention: +, -, * = 1, FMA = 2; div/sqrt/transcendentals are not counted.
+# Integration 56 FLOP per RK4 step, +8 while the ray has not hit anything (normalise + length
+# of the segment); the four field evaluations per step: swirl 48, lens 64, hot-air 217
+# (temperature at p_prev once per step), warp 24; one Moeller-Trumbore test with stored
+# edges 44.
+FLOP_INTEGRATION = 56
+FLOP_PRE_HIT = 8
+FLOP_FIELD4 = {"swirl": 48, "lens": 64, "hot_air": 217, "warp": 24}
+FLOP_TRIANGLE = 44
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_flops(field: str, c: dict) -> float:
+    pre = c["queries"] + c["skips"]
+    return (c["rk4_steps"] * (FLOP_INTEGRATION + FLOP_FIELD4[field]) + pre * FLOP_PRE_HIT
+            + c["triangle_tests"] * FLOP_TRIANGLE)
+
+
+def cpu_baseline(wl, verts, faces, budget_s: float) -> dict:
+    """FP32 CPU restatement on a strided sample of each field's pixels."""
+    from oracle import oracle as O
+
+    threads = max(1, min(16, O.threads_available()))
+    cam = O.as_oracle_struct(wl.camera(), O.CameraIn)
+    st = O.as_oracle_struct(wl.settings(), O.SettingsIn)
+    ig = O.as_oracle_struct(wl.integrator(), O.IntegratorIn)
+    W, H = wl.width, wl.height
+    gx, gy = np.meshgrid(np.arange(W), np.arange(H))
+    allpix = np.stack([gx.ravel(), gy.ravel()], 1).astype(np.int32)
+    rng = np.random.default_rng(0)
+    order = rng.permutation(len(allpix))
+    # pilot: per-ray cost
+    pilot = allpix[order[: 4 * threads]]
+    t0 = time.perf_counter()
+    for name in wl.fields:
+        O.render_pixels(cam, st, ig, L.FIELDS[name], verts, faces, W, H, pilot, threads)
+    per_ray = (time.perf_counter() - t0) / (len(pilot) * len(wl.fields))
+    n = int(max(threads, min(len(allpix), budget_s / max(per_ray, 1e-9) / len(wl.fields))))
+    sample = allpix[order[:n]]
+    steps = 0
+    t0 = time.perf_counter()
+    for name in wl.fields:
+        r = O.render_pixels(cam, st, ig, L.FIELDS[name], verts, faces, W, H, sample, threads)
+        steps += r["rk4_steps"]
+    dt = time.perf_counter() - t0
+    return {"value": steps / dt, "unit": "RK4 ray-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} random pixels (seed 0) of each of the 4 fields at 1920x1080, FP32 CPU restatement "
+                      f"of main_view.wgsl (brute force over {len(faces)} faces per pre-hit step), "
+                      f"{steps} RK4 steps in {dt:.1f} s on {threads} threads",
+            "frame_ms_extrapolated": dt / (n * len(wl.fields)) * W * H * 1e3}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the ray-cast pass has no multi-GPU path in this build "
+              "(rays are independent; sharding by image tiles or views is described in DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    wl = configs.CONFIG2
+    verts, faces = wl.mesh()
+    view = L.MainView(wl.width, wl.height)
+    view.update_model(verts, faces)
+    view.update_camera(wl.camera())
+    view.update_settings(wl.settings())
+    view.set_integrator(wl.integrator())
+
+    # warm-up also records each field's (deterministic) work counters
+    counters = {}
+    for w in range(max(1, a.warmup)):
+        for name in wl.fields:
+            view.set_field(name)
+            view.render()
+            counters[name] = view.counters()
+        log(f"warmup {w + 1}/{max(1, a.warmup)} done")
+
+    view.synchronize()
+    view.reset_timings()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        for name in wl.fields:
+            view.set_field(name)
+            view.render()
+    view.synchronize()
+    elapsed = time.perf_counter() - t0
+    ray_ms, pass_ms = view.timings()
+    log(f"timed {a.steps} steps in {elapsed:.3f} s")
+
+    nf = len(wl.fields)
+    assert len(ray_ms) == a.steps * nf
+    per_field = {}
+    for i, name in enumerate(wl.fields):
+        rms = float(np.mean(ray_ms[i::nf]))
+        pms = float(np.mean(pass_ms[i::nf]))
+        c = counters[name]
+        fl = kernel_flops(name, c)
+        per_field[name] = {"raycast_ms": rms, "frame_ms": pms, "rk4_steps": c["rk4_steps"],
+                           "triangle_tests": c["triangle_tests"], "pre_hit_steps": c["queries"] + c["skips"],
+                           "flop": fl, "tflops": fl / (rms * 1e-3) / 1e12,
+                           "gsteps_per_s": c["rk4_steps"] / (rms * 1e-3) / 1e9}
+    steps_per_frame_set = sum(counters[n]["rk4_steps"] for n in wl.fields)
+    value = steps_per_frame_set * a.steps / elapsed
+    dom = max(per_field, key=lambda n: per_field[n]["raycast_ms"])
+    d = per_field[dom]
+    kname = {"swirl": "FieldSwirl", "lens": "FieldLens", "hot_air": "FieldHotAir", "warp": "FieldWarp"}[dom]
+    roofline = {"bound": "mfma", "kernel": f"void linon::raycast_kernel<lw::{kname}>(RaycastParams)",
+                "note": "compute-bound on FP32 vector ALUs (no matrix cores); peak = MI355X FP32 157.3 TFLOP/s",
+                "achieved": d["tflops"],
+                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": d["tflops"] / PEAK_FP32_TFLOPS,
+                "traffic": None,
+                "flop_per_launch": d["flop"], "launch_ms": d["raycast_ms"],
+                "all_raycast_kernels_tflops": sum(p["flop"] for p in per_field.values())
+                / (sum(p["raycast_ms"] for p in per_field.values()) * 1e-3) / 1e12}
+    traffic_file = ROOT / "profiles" / "traffic.json"
+    if traffic_file.exists():
+        try:
+            t = json.loads(traffic_file.read_text()).get(dom)
+            if t:
+                roofline["traffic"] = t
+        except (ValueError, OSError):
+            pass
+
+    out = {
+        "metric": "RK4 ray-steps/s + 1080p frame ms; surrogate vs plain RK4 at <=1e-3 exit error",
+        "value": value,
+        "unit": "RK4 ray-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": elapsed / a.steps * 1e3,
+        "frame_ms_1080p": float(np.mean([p["frame_ms"] for p in per_field.values()])),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "width": wl.width, "height": wl.height, "fields": list(wl.fields),
+                   "h": wl.h, "t_max": wl.t_max, "max_steps": wl.max_steps, "adaptive": wl.adaptive,
+                   "overlay": wl.overlay, "faces": int(len(faces)), "sample_rays": wl.samples,
+                   "step": "one 1080p frame per field (4 frames)"},
+        "per_field": per_field,
+        "roofline": roofline,
+    }
+    view.close()
+    if not a.no_cpu_baseline:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(wl, verts, faces, a.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,200 @@
+/*
+ * linon.h — C ABI of the MI355X (gfx950) nonlinear ray-cast pass.
+ *
+ * One context stands in for the compute pass that linon's `MainView` owns
+ * (src/main_view.rs:53-79): the ray-cast kernel of src/main_view.wgsl and the Lyapunov
+ * overlay of src/overlay.wgsl, with the uniform and storage layouts those shaders bind.
+ * Every entry point below names the reference interface it replaces.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only; `stream` arguments are hipStream_t passed as void*
+ *    (NULL = the context's own stream).
+ *  - Every function returns a linon_status; on error the context keeps a message that
+ *    linon_last_error() returns (the reference reports shader errors the same way, as a
+ *    string, src/main_view.rs:657-679).  Nothing throws or aborts across the ABI.
+ *  - A context is not thread-safe (as MainView is not); rendering is stream-ordered and
+ *    asynchronous; the linon_read_* calls synchronise.
+ *  - Image buffers are row-major with texel row 0 at the top, exactly as wgpu stores the
+ *    reference's textures: the ray of invocation (gx, gy) lands in texel (gx, H-1-gy)
+ *    (src/main_view.wgsl:342).
+ */
+#ifndef LINON_H
+#define LINON_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LINON_ABI_VERSION 1
+
+typedef enum linon_status {
+    LINON_OK = 0,
+    LINON_ERR_ARGUMENT = 1,   /* bad pointer, size or enum value */
+    LINON_ERR_COMPILE = 2,    /* field body rejected by the translator or by hiprtc */
+    LINON_ERR_HIP = 3,        /* a HIP runtime call failed */
+    LINON_ERR_STATE = 4,      /* call order violated (e.g. render before a mesh is set) */
+    LINON_ERR_UNSUPPORTED = 5
+} linon_status;
+
+/* CameraUniform, src/main_view.rs:12-19 == struct Camera, src/main_view.wgsl:7-12 (112 B).
+ * view_direction[3] is 0 in the reference; here a positive value is a vertical field of view
+ * in degrees (the pinhole of the benchmark configurations), 0 keeps the reference's fixed
+ * 0.025 x aspect viewport at focal length 0.035 (src/main_view.wgsl:355-357). */
+typedef struct linon_camera {
+    float origin[4];
+    float view_direction[4];
+    float up[4];
+    float view_matrix[16]; /* carried for layout only; the compute pass does not read it */
+} linon_camera;
+
+/* Settings, src/main_view.rs:43-51 == struct Settings, src/main_view.wgsl:16-23 (24 B). */
+typedef struct linon_settings {
+    float field_weight;
+    float mouse_pos[2];
+    int32_t overlay_mode;             /* 0 = disabled, 1 = Lyapunov exponents */
+    int32_t central_difference_delta; /* 1..10 in the reference GUI */
+    float lyapunov_scaling;
+} linon_settings;
+
+/* Integrator knobs.  The reference fixes them in the WGSL text (h_initial :241, the
+ * `t <= 5.0` loop :244, adaptive_sampling :227, the sample-ray trace :287-293) and flips
+ * them by string replacement (src/main_view.rs:752-829). */
+typedef enum linon_mode {
+    LINON_MODE_REFERENCE = 0, /* for (; t <= t_max;) as main_view.wgsl:244, max_steps as a cap */
+    LINON_MODE_ESCAPE = 1     /* stop when |p| > escape_radius moving outward, at max_steps,
+                                 or at the first hit unless need_mapping is set */
+} linon_mode;
+
+typedef struct linon_integrator {
+    float h_initial;          /* reference 0.1; "Enhance" renders once at 0.001 */
+    float t_max;              /* reference 5.0 */
+    int32_t adaptive;         /* reference: always on (App. note: the toggle is a no-op) */
+    int32_t max_steps;        /* cap on RK4 iterations per ray (adaptive redos included) */
+    int32_t mode;             /* linon_mode */
+    float escape_radius;      /* escape mode: bounding-sphere radius */
+    int32_t samples_per_ray;  /* ray_samples records per sample ray, reference 100 */
+    int32_t sample_steps;     /* RK4 steps of each sample ray, reference 5000 */
+    int32_t need_mapping;     /* escape mode: keep integrating after a hit */
+    int32_t pad[3];
+} linon_integrator;
+
+/* Field ids: 1..9 follow PredefinedFunction (src/functions.rs:2-13); 0 (Custom) has no
+ * predefined body.  16 is the shader's stub body `return v;` (src/main_view.wgsl:148).
+ * 17..20 are the benchmark fields written with the same helpers. */
+typedef enum linon_field {
+    LINON_FIELD_CUSTOM = 0,
+    LINON_FIELD_MIRAGE_SPHERICAL = 1,
+    LINON_FIELD_MIRAGE_PLANE = 2,
+    LINON_FIELD_MIRAGE_SPHERICAL_SIGMOID = 3,
+    LINON_FIELD_MIRAGE_PLANE_SIGMOID = 4,
+    LINON_FIELD_TRANSLATION_X = 5,
+    LINON_FIELD_TRANSLATION_Z = 6,
+    LINON_FIELD_ROTATION = 7,
+    LINON_FIELD_LORENZ = 8,
+    LINON_FIELD_ROESSLER = 9,
+    LINON_FIELD_IDENTITY = 16,
+    LINON_FIELD_SWIRL = 17,    /* rotateY(v0, 0.5 * length(p)) */
+    LINON_FIELD_LENS = 18,     /* v - 0.3 * p / max(|p|^3, 1e-3) */
+    LINON_FIELD_HOT_AIR = 19,  /* refraction(T(p_prev), T(p), v, normalize(p - (0,-0.5,0))) */
+    LINON_FIELD_WARP = 20      /* v0 + 0.3 * (sin 4p.y, 0, sin 4p.x) */
+} linon_field;
+
+/* Scene acceleration: the reference tests every face for every step (src/main_view.wgsl:182).
+ * AUTO picks the uniform grid (cell lists + distance field) for meshes above a few dozen
+ * faces; BRUTE keeps the reference's loop over all faces (faces staged in LDS). */
+typedef enum linon_accel {
+    LINON_ACCEL_AUTO = 0,
+    LINON_ACCEL_BRUTE = 1,
+    LINON_ACCEL_GRID = 2
+} linon_accel;
+
+typedef struct linon_counters {
+    uint64_t rk4_steps;       /* RK4 iterations (4 field evaluations each) */
+    uint64_t triangle_tests;  /* Moeller-Trumbore tests */
+    uint64_t cell_visits;     /* grid cells walked by segment queries */
+    uint64_t queries;         /* segment queries that reached the triangle stage */
+    uint64_t skips;           /* pre-hit segments the distance bound proved empty */
+    uint64_t rays;
+    float last_render_ms;     /* device time of the last linon_render (main + overlay) */
+    float last_raycast_ms;    /* device time of the ray-cast kernel alone */
+} linon_counters;
+
+/* Device-resident outputs, for zero-copy hand-off (e.g. external-memory interop). */
+typedef struct linon_device_buffers {
+    void* final_rgba8;     /* W*H*4 B   — MainView::texture (overlay output) */
+    void* ray_casting_rgba8; /* W*H*4 B — ray_casting_texture */
+    void* mapping_rgba32f; /* W*H*16 B  — mapping_texture */
+    void* exponents_f32;   /* W*H*4 B   — exponents_buffer (row stride = H, as overlay.wgsl:128) */
+    void* lyapunov_f32;    /* W*H*4 B   — lambda per texel, row stride W (-1 where not computed) */
+    void* hit_face_i32;    /* W*H*4 B   — first-hit face index per texel (-1 = none) */
+    void* steps_i32;       /* W*H*4 B   — RK4 iterations per texel */
+    void* ray_samples;     /* 8*samples_per_ray*32 B — RaySamples (src/main_view.wgsl:55-63) */
+    uint32_t width, height;
+} linon_device_buffers;
+
+typedef struct linon_context linon_context;
+
+/* MainView::new (src/main_view.rs:82-458): allocates the outputs for a width x height view,
+ * sets the reference's initial settings (:133-139), integrator (h 0.1, t 5, adaptive) and
+ * field (MirageSphericalSigmoid, :100).  device = HIP device ordinal. */
+linon_status linon_create(linon_context** out, int device, uint32_t width, uint32_t height);
+void linon_destroy(linon_context* ctx);
+const char* linon_last_error(const linon_context* ctx);
+int linon_abi_version(void);
+
+/* MainView::resize_texture (src/main_view.rs:552-647). */
+linon_status linon_resize(linon_context* ctx, uint32_t width, uint32_t height);
+/* MainView::update_camera (src/main_view.rs:490-494). */
+linon_status linon_set_camera(linon_context* ctx, const linon_camera* camera);
+/* MainView::update_settings (src/main_view.rs:547-550). */
+linon_status linon_set_settings(linon_context* ctx, const linon_settings* settings);
+/* MainView::update_model (src/main_view.rs:460-488): the vertex/face storage buffers of
+ * bind group 1.  Copied to the device; the acceleration structure is built here.  The face
+ * index (position in `faces`) is the hit id and the tie-break key. */
+linon_status linon_set_mesh(linon_context* ctx, const float* xyz, uint32_t num_vertices,
+                            const uint32_t* faces, uint32_t num_faces);
+linon_status linon_set_accel(linon_context* ctx, int accel, int grid_resolution);
+/* MainView::reload_shader with a body from PredefinedFunction::to_code (src/functions.rs:33-128). */
+linon_status linon_set_field_predefined(linon_context* ctx, int field);
+/* MainView::reload_shader(None, body) (src/main_view.rs:649-690): a WGSL field body, spliced
+ * into `fn field_function(p_prev, p, v0, v, t) -> vec3<f32>` (with_field_function,
+ * :852-861), translated to HIP and compiled with hiprtc.  LINON_ERR_COMPILE + message on a
+ * bad body; the previous field stays active. */
+linon_status linon_set_field_wgsl(linon_context* ctx, const char* body);
+linon_status linon_set_integrator(linon_context* ctx, const linon_integrator* integrator);
+linon_status linon_get_integrator(const linon_context* ctx, linon_integrator* integrator);
+
+/* MainView::render (src/main_view.rs:831-849): the ray-cast dispatch, then the overlay. */
+linon_status linon_render(linon_context* ctx, void* stream);
+/* MainView::render_high_accuracy (src/main_view.rs:778-802): one render at h_initial = 0.001. */
+linon_status linon_render_high_accuracy(linon_context* ctx, void* stream);
+linon_status linon_synchronize(linon_context* ctx);
+
+/* Read-back (the reference keeps these on the GPU; sizes are checked against W*H). */
+linon_status linon_read_final(linon_context* ctx, uint8_t* rgba, size_t bytes);
+linon_status linon_read_ray_casting(linon_context* ctx, uint8_t* rgba, size_t bytes);
+linon_status linon_read_mapping(linon_context* ctx, float* xyzw, size_t bytes);
+linon_status linon_read_exponents(linon_context* ctx, float* data, size_t bytes);
+linon_status linon_read_lyapunov(linon_context* ctx, float* data, size_t bytes);
+linon_status linon_read_hit_faces(linon_context* ctx, int32_t* data, size_t bytes);
+linon_status linon_read_steps(linon_context* ctx, int32_t* data, size_t bytes);
+linon_status linon_read_ray_samples(linon_context* ctx, float* data, size_t bytes);
+linon_status linon_get_counters(linon_context* ctx, linon_counters* out);
+/* Device time of every render since the last reset (the newest LINON_TIMING_SLOTS kept), from
+ * HIP events recorded on the render's stream: per render the ray-cast kernel and the whole
+ * pass (ray cast + overlay), in ms.  Synchronises.  (The reference has no timing:
+ * timestamp_writes is None, src/main_view.rs:832-835.) */
+#define LINON_TIMING_SLOTS 256
+linon_status linon_get_timings(linon_context* ctx, float* raycast_ms, float* render_ms, int capacity,
+                               int* count);
+linon_status linon_reset_timings(linon_context* ctx);
+linon_status linon_get_device_buffers(linon_context* ctx, linon_device_buffers* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LINON_H */

@@ -1,0 +1,362 @@
+"""linon_amd — MI355X (gfx950) nonlinear ray casting behind linon's MainView interface.
+
+The compute pass of linon (src/main_view.wgsl ray cast + src/overlay.wgsl Lyapunov overlay)
+runs as hand-written HIP kernels in ``liblinon_hip.so``, which exports the C ABI declared in
+``include/linon.h``.  This module binds that ABI with ctypes and mirrors the reference's
+``MainView`` (src/main_view.rs:53-850) method for method, so that host code written against
+the reference reads the same here:
+
+    view = MainView(width, height)                       # MainView::new
+    view.update_model(verts, faces)                       # MainView::update_model
+    view.update_camera(camera)                            # MainView::update_camera
+    view.update_settings(Settings(...))                   # MainView::update_settings
+    view.reload_shader(field_body)                        # MainView::reload_shader (WGSL body)
+    view.render()                                         # MainView::render
+    view.render_high_accuracy()                           # MainView::render_high_accuracy
+
+There is no CPU fallback: importing works without a GPU, but creating a view raises when the
+library or a HIP device is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field as dc_field
+from pathlib import Path
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "liblinon_hip.so"
+
+LINON_OK = 0
+STATUS_NAMES = {0: "OK", 1: "ARGUMENT", 2: "COMPILE", 3: "HIP", 4: "STATE", 5: "UNSUPPORTED"}
+
+MODE_REFERENCE = 0
+MODE_ESCAPE = 1
+ACCEL_AUTO, ACCEL_BRUTE, ACCEL_GRID = 0, 1, 2
+
+# Field ids (include/linon.h): 1..9 = PredefinedFunction order of src/functions.rs:2-13.
+FIELDS = {
+    "custom": 0,
+    "mirage_spherical": 1,
+    "mirage_plane": 2,
+    "mirage_spherical_sigmoid": 3,
+    "mirage_plane_sigmoid": 4,
+    "translation_x": 5,
+    "translation_z": 6,
+    "rotation": 7,
+    "lorenz": 8,
+    "roessler": 9,
+    "identity": 16,
+    "swirl": 17,
+    "lens": 18,
+    "hot_air": 19,
+    "warp": 20,
+}
+PREDEFINED_FIELD_IDS = [1, 2, 3, 4, 5, 6, 7, 8, 9, 16, 17, 18, 19, 20]
+BENCH_FIELDS = ("swirl", "lens", "hot_air", "warp")
+
+
+class LinonError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"linon error {STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+class Camera(C.Structure):
+    """CameraUniform (src/main_view.rs:12-19), 112 bytes.  view_direction[3] > 0 is a
+    vertical field of view in degrees; 0 keeps the reference's fixed viewport."""
+    _fields_ = [("origin", C.c_float * 4), ("view_direction", C.c_float * 4), ("up", C.c_float * 4),
+                ("view_matrix", C.c_float * 16)]
+
+    @classmethod
+    def make(cls, origin, view_direction, up, fov_deg: float = 0.0) -> "Camera":
+        c = cls()
+        for i in range(3):
+            c.origin[i] = float(origin[i])
+            c.view_direction[i] = float(view_direction[i])
+            c.up[i] = float(up[i])
+        c.view_direction[3] = float(fov_deg)
+        for i in range(4):
+            c.view_matrix[5 * i] = 1.0
+        return c
+
+    @classmethod
+    def arcball_default(cls, center) -> "Camera":
+        """ArcballCamera::new(center, 1, ..) followed by zoom(-1, 1) (src/main_view.rs:153-155,
+        src/arcball.rs:33-46,91-98): eye at center + (0, 0, 2), looking down -z, y up.  The
+        eye is computed in f32 as cgmath does (center + 2)."""
+        c = np.asarray(center, dtype=np.float32)
+        eye = np.array([c[0], c[1], c[2] + np.float32(2.0)], dtype=np.float32)
+        return cls.make(eye, (0.0, 0.0, -1.0), (0.0, 1.0, 0.0))
+
+    @classmethod
+    def look_at(cls, eye, target, up_hint=(0.0, 1.0, 0.0), fov_deg: float = 60.0) -> "Camera":
+        eye = np.asarray(eye, dtype=np.float64)
+        d = np.asarray(target, dtype=np.float64) - eye
+        d /= np.linalg.norm(d)
+        up = np.asarray(up_hint, dtype=np.float64)
+        if abs(float(np.dot(up, d))) > 0.999:
+            up = np.array([1.0, 0.0, 0.0]) if abs(d[0]) < 0.9 else np.array([0.0, 0.0, 1.0])
+        right = np.cross(d, up)
+        right /= np.linalg.norm(right)
+        up = np.cross(right, d)
+        return cls.make(eye, d, up, fov_deg)
+
+
+class Settings(C.Structure):
+    """Settings (src/main_view.rs:43-51), 24 bytes."""
+    _fields_ = [("field_weight", C.c_float), ("mouse_pos", C.c_float * 2), ("overlay_mode", C.c_int32),
+                ("central_difference_delta", C.c_int32), ("lyapunov_scaling", C.c_float)]
+
+    @classmethod
+    def make(cls, field_weight=1.0, mouse_pos=(0.5, 0.6), overlay_mode=0, central_difference_delta=1,
+             lyapunov_scaling=50.0) -> "Settings":
+        s = cls()
+        s.field_weight = field_weight
+        s.mouse_pos[0], s.mouse_pos[1] = mouse_pos
+        s.overlay_mode = overlay_mode
+        s.central_difference_delta = central_difference_delta
+        s.lyapunov_scaling = lyapunov_scaling
+        return s
+
+
+class Integrator(C.Structure):
+    _fields_ = [("h_initial", C.c_float), ("t_max", C.c_float), ("adaptive", C.c_int32),
+                ("max_steps", C.c_int32), ("mode", C.c_int32), ("escape_radius", C.c_float),
+                ("samples_per_ray", C.c_int32), ("sample_steps", C.c_int32), ("need_mapping", C.c_int32),
+                ("pad", C.c_int32 * 3)]
+
+    @classmethod
+    def make(cls, h_initial=0.1, t_max=5.0, adaptive=True, max_steps=100000, mode=MODE_REFERENCE,
+             escape_radius=2.0, samples_per_ray=100, sample_steps=5000, need_mapping=False) -> "Integrator":
+        i = cls()
+        i.h_initial, i.t_max, i.adaptive, i.max_steps = h_initial, t_max, int(adaptive), max_steps
+        i.mode, i.escape_radius = mode, escape_radius
+        i.samples_per_ray, i.sample_steps, i.need_mapping = samples_per_ray, sample_steps, int(need_mapping)
+        return i
+
+
+class Counters(C.Structure):
+    _fields_ = [("rk4_steps", C.c_uint64), ("triangle_tests", C.c_uint64), ("cell_visits", C.c_uint64),
+                ("queries", C.c_uint64), ("skips", C.c_uint64), ("rays", C.c_uint64), ("last_render_ms", C.c_float),
+                ("last_raycast_ms", C.c_float)]
+
+
+class DeviceBuffers(C.Structure):
+    _fields_ = [("final_rgba8", C.c_void_p), ("ray_casting_rgba8", C.c_void_p), ("mapping_rgba32f", C.c_void_p),
+                ("exponents_f32", C.c_void_p), ("lyapunov_f32", C.c_void_p), ("hit_face_i32", C.c_void_p),
+                ("steps_i32", C.c_void_p), ("ray_samples", C.c_void_p), ("width", C.c_uint32),
+                ("height", C.c_uint32)]
+
+
+assert C.sizeof(Camera) == 112 and C.sizeof(Settings) == 24 and C.sizeof(Integrator) == 48
+
+_lib = None
+
+# Every symbol include/linon.h declares, with its ctypes signature.
+_P = C.c_void_p
+SIGNATURES = {
+    "linon_abi_version": (C.c_int, []),
+    "linon_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_uint32, C.c_uint32]),
+    "linon_destroy": (None, [_P]),
+    "linon_last_error": (C.c_char_p, [_P]),
+    "linon_resize": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
+    "linon_set_camera": (C.c_int, [_P, C.POINTER(Camera)]),
+    "linon_set_settings": (C.c_int, [_P, C.POINTER(Settings)]),
+    "linon_set_mesh": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32]),
+    "linon_set_accel": (C.c_int, [_P, C.c_int, C.c_int]),
+    "linon_set_field_predefined": (C.c_int, [_P, C.c_int]),
+    "linon_set_field_wgsl": (C.c_int, [_P, C.c_char_p]),
+    "linon_set_integrator": (C.c_int, [_P, C.POINTER(Integrator)]),
+    "linon_get_integrator": (C.c_int, [_P, C.POINTER(Integrator)]),
+    "linon_render": (C.c_int, [_P, _P]),
+    "linon_render_high_accuracy": (C.c_int, [_P, _P]),
+    "linon_synchronize": (C.c_int, [_P]),
+    "linon_read_final": (C.c_int, [_P, _P, C.c_size_t]),
+    "linon_read_ray_casting": (C.c_int, [_P, _P, C.c_size_t]),
+    "linon_read_mapping": (C.c_int, [_P, _P, C.c_size_t]),
+    "linon_read_exponents": (C.c_int, [_P, _P, C.c_size_t]),
+    "linon_read_lyapunov": (C.c_int, [_P, _P, C.c_size_t]),
+    "linon_read_hit_faces": (C.c_int, [_P, _P, C.c_size_t]),
+    "linon_read_steps": (C.c_int, [_P, _P, C.c_size_t]),
+    "linon_read_ray_samples": (C.c_int, [_P, _P, C.c_size_t]),
+    "linon_get_counters": (C.c_int, [_P, C.POINTER(Counters)]),
+    "linon_get_device_buffers": (C.c_int, [_P, C.POINTER(DeviceBuffers)]),
+    "linon_get_timings": (C.c_int, [_P, _P, _P, C.c_int, C.POINTER(C.c_int)]),
+    "linon_reset_timings": (C.c_int, [_P]),
+}
+
+
+def library_path() -> Path:
+    return _LIB_PATH
+
+
+def load_library() -> C.CDLL:
+    """Loads liblinon_hip.so (raises if it has not been built; there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        raise ImportError(f"{_LIB_PATH} is missing: build it with `python -m linon_amd.build`")
+    lib = C.CDLL(str(_LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+@dataclass
+class Frame:
+    """Read-back of one render (texel row 0 at the top, as the reference's textures)."""
+    final: np.ndarray        # (H, W, 4) uint8 — MainView::texture
+    ray_casting: np.ndarray  # (H, W, 4) uint8
+    mapping: np.ndarray      # (H, W, 4) float32
+    hit_faces: np.ndarray    # (H, W) int32
+    steps: np.ndarray        # (H, W) int32
+    lyapunov: np.ndarray     # (H, W) float32
+    exponents: np.ndarray    # (W*H,) float32, row stride H as src/overlay.wgsl:128
+    counters: dict = dc_field(default_factory=dict)
+
+
+class MainView:
+    """Mirror of linon's MainView (src/main_view.rs:53-850) over the C ABI."""
+
+    def __init__(self, width: int, height: int, device: int = 0):
+        self._lib = load_library()
+        h = C.c_void_p()
+        st = self._lib.linon_create(C.byref(h), device, width, height)
+        if st != LINON_OK:
+            raise LinonError(st, f"linon_create({device}, {width}, {height}) failed (is a GPU visible?)")
+        self._h = h
+        self.width, self.height = width, height
+
+    # -- lifecycle ---------------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.linon_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st: int) -> None:
+        if st != LINON_OK:
+            msg = self._lib.linon_last_error(self._h)
+            raise LinonError(st, msg.decode() if msg else "")
+
+    # -- MainView interface ------------------------------------------------------------
+    def resize_texture(self, width: int, height: int) -> None:
+        self._check(self._lib.linon_resize(self._h, width, height))
+        self.width, self.height = width, height
+
+    def update_camera(self, camera: Camera) -> None:
+        self._check(self._lib.linon_set_camera(self._h, C.byref(camera)))
+
+    def update_settings(self, settings: Settings) -> None:
+        self._check(self._lib.linon_set_settings(self._h, C.byref(settings)))
+
+    def update_model(self, verts, faces) -> None:
+        v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+        f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+        self._check(self._lib.linon_set_mesh(self._h, _ptr(v), len(v), _ptr(f), len(f)))
+
+    def set_accel(self, accel: int = ACCEL_AUTO, grid_resolution: int = 0) -> None:
+        self._check(self._lib.linon_set_accel(self._h, accel, grid_resolution))
+
+    def set_field(self, field) -> None:
+        fid = FIELDS[field] if isinstance(field, str) else int(field)
+        self._check(self._lib.linon_set_field_predefined(self._h, fid))
+
+    def reload_shader(self, field_body: str) -> None:
+        """MainView::reload_shader(None, body): compile a WGSL field body."""
+        self._check(self._lib.linon_set_field_wgsl(self._h, field_body.encode()))
+
+    def set_integrator(self, integ: Integrator) -> None:
+        self._check(self._lib.linon_set_integrator(self._h, C.byref(integ)))
+
+    def integrator(self) -> Integrator:
+        i = Integrator()
+        self._check(self._lib.linon_get_integrator(self._h, C.byref(i)))
+        return i
+
+    def render(self, stream: int | None = None) -> None:
+        self._check(self._lib.linon_render(self._h, C.c_void_p(stream)))
+
+    def render_high_accuracy(self, stream: int | None = None) -> None:
+        self._check(self._lib.linon_render_high_accuracy(self._h, C.c_void_p(stream)))
+
+    def synchronize(self) -> None:
+        self._check(self._lib.linon_synchronize(self._h))
+
+    # -- read-back ----------------------------------------------------------------------
+    def _read(self, fn, shape, dtype) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        self._check(fn(self._h, _ptr(out), out.nbytes))
+        return out
+
+    def read_final(self) -> np.ndarray:
+        return self._read(self._lib.linon_read_final, (self.height, self.width, 4), np.uint8)
+
+    def read_ray_casting(self) -> np.ndarray:
+        return self._read(self._lib.linon_read_ray_casting, (self.height, self.width, 4), np.uint8)
+
+    def read_mapping(self) -> np.ndarray:
+        return self._read(self._lib.linon_read_mapping, (self.height, self.width, 4), np.float32)
+
+    def read_hit_faces(self) -> np.ndarray:
+        return self._read(self._lib.linon_read_hit_faces, (self.height, self.width), np.int32)
+
+    def read_steps(self) -> np.ndarray:
+        return self._read(self._lib.linon_read_steps, (self.height, self.width), np.int32)
+
+    def read_lyapunov(self) -> np.ndarray:
+        return self._read(self._lib.linon_read_lyapunov, (self.height, self.width), np.float32)
+
+    def read_exponents(self) -> np.ndarray:
+        return self._read(self._lib.linon_read_exponents, (self.width * self.height,), np.float32)
+
+    def read_ray_samples(self) -> np.ndarray:
+        n = self.integrator().samples_per_ray
+        return self._read(self._lib.linon_read_ray_samples, (8, max(1, n), 2, 4), np.float32)
+
+    def counters(self) -> dict:
+        c = Counters()
+        self._check(self._lib.linon_get_counters(self._h, C.byref(c)))
+        return {k: getattr(c, k) for k, _ in Counters._fields_}
+
+    def timings(self):
+        """(raycast_ms, render_ms) arrays of the renders since reset_timings()."""
+        a = np.zeros(256, np.float32)
+        b = np.zeros(256, np.float32)
+        n = C.c_int(0)
+        self._check(self._lib.linon_get_timings(self._h, _ptr(a), _ptr(b), 256, C.byref(n)))
+        return a[: n.value].copy(), b[: n.value].copy()
+
+    def reset_timings(self) -> None:
+        self._check(self._lib.linon_reset_timings(self._h))
+
+    def device_buffers(self) -> DeviceBuffers:
+        d = DeviceBuffers()
+        self._check(self._lib.linon_get_device_buffers(self._h, C.byref(d)))
+        return d
+
+    def read_frame(self) -> Frame:
+        return Frame(self.read_final(), self.read_ray_casting(), self.read_mapping(), self.read_hit_faces(),
+                     self.read_steps(), self.read_lyapunov(), self.read_exponents(), self.counters())

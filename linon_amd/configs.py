@@ -1,0 +1,69 @@
+"""The benchmark workloads as host-side set-ups of a MainView.
+
+* ``config1``: 256x256, 1 ray/pixel, 60-degree pinhole at distance 3 looking at the origin,
+  the Cornell box (centred), swirl field, h = 0.01, at most 1000 RK4 iterations, no overlay.
+* ``config2``: 1920x1080, the Cornell box plus a seeded 20,480-face displaced icosphere,
+  fields swirl / lens / hot-air / warp each rendered once, h = 0.005, at most 4000 RK4
+  iterations, Lyapunov overlay over 3x3 neighbours (delta 1, scaling 50).
+
+Both keep the reference's integration loop `for (; t <= 5.0;)` with `t = t + h` in f32
+(src/main_view.wgsl:244,265); the step cap never binds at these h.  They integrate with the
+fixed step h: the reference's adaptive rule (src/main_view.wgsl:251-255,267-269) compares
+|V - last_V| against 10x its previous value, and wherever the field is uniform both sides are
+rounding noise, so which steps it refines is decided by the last bits of the arithmetic.
+The adaptive path is exercised by its own tests (interactive default h = 0.1).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+from . import FIELDS, MODE_REFERENCE, Camera, Integrator, Settings
+from . import scenes
+
+
+@dataclass
+class Workload:
+    name: str
+    width: int
+    height: int
+    fields: tuple
+    h: float
+    max_steps: int
+    overlay: bool
+    level: int | None  # icosphere subdivision level (None: Cornell box only)
+    t_max: float = 5.0
+    adaptive: bool = False   # fixed-step RK4 ("step h = ..."); see the module docstring
+    samples: bool = True
+
+    def camera(self) -> Camera:
+        return Camera.look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), fov_deg=60.0)
+
+    def settings(self) -> Settings:
+        return Settings.make(field_weight=1.0, mouse_pos=(0.5, 0.6), overlay_mode=1 if self.overlay else 0,
+                             central_difference_delta=1, lyapunov_scaling=50.0)
+
+    def integrator(self) -> Integrator:
+        return Integrator.make(h_initial=self.h, t_max=self.t_max, adaptive=self.adaptive,
+                               max_steps=self.max_steps, mode=MODE_REFERENCE,
+                               samples_per_ray=100 if self.samples else 0,
+                               sample_steps=5000 if self.samples else 0)
+
+    def mesh(self):
+        if self.level is None:
+            return scenes.cornell_box_centered()
+        return scenes.benchmark_scene(self.level)
+
+    def field_ids(self):
+        return [FIELDS[f] for f in self.fields]
+
+
+CONFIG1 = Workload("plain-rk4-256-swirl-cornell", 256, 256, ("swirl",), 0.01, 1000, False, None)
+CONFIG2 = Workload("plain-rk4-overlay-1080p-4fields-cornell+ico20k", 1920, 1080,
+                   ("swirl", "lens", "hot_air", "warp"), 0.005, 4000, True, 5)
+
+
+def setup_view(view, wl: Workload, field) -> None:
+    view.update_camera(wl.camera())
+    view.update_settings(wl.settings())
+    view.set_integrator(wl.integrator())
+    view.set_field(field)

@@ -1,0 +1,538 @@
+// C-ABI context: the device-side state of linon's MainView compute pass (textures, uniforms,
+// mesh buffers, the selected field) and the dispatch of the ray-cast and overlay kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/linon.h"
+#include "context.hpp"
+#include "grid.hpp"
+#include "kernels.hpp"
+
+static_assert(sizeof(linon_camera) == 112, "CameraUniform is 112 bytes (src/main_view.rs:12-19)");
+static_assert(sizeof(linon_settings) == 24, "Settings is 24 bytes (src/main_view.rs:43-51)");
+static_assert(sizeof(linon_integrator) == 48, "linon_integrator layout");
+
+namespace linon {
+
+#define LN_CHECK_HIP(ctx, call)                                                           \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            return (ctx)->fail(LINON_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+        }                                                                                 \
+    } while (0)
+
+linon_status Context::fail(linon_status s, const std::string& msg) {
+    err = msg;
+    return s;
+}
+
+void Context::free_images() {
+    for (void** p : {(void**)&d_color, (void**)&d_final, (void**)&d_mapping, (void**)&d_exponents,
+                     (void**)&d_lyap, (void**)&d_hit, (void**)&d_steps}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+}
+
+void Context::free_mesh() {
+    for (void** p : {(void**)&d_tris, (void**)&d_cell_start, (void**)&d_cell_faces, (void**)&d_dist}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    num_faces = 0;
+    have_mesh = false;
+}
+
+linon_status Context::alloc_images(uint32_t w, uint32_t h) {
+    free_images();
+    width = w;
+    height = h;
+    const size_t n = (size_t)w * h;
+    LN_CHECK_HIP(this, hipMalloc(&d_color, n * 4));
+    LN_CHECK_HIP(this, hipMalloc(&d_final, n * 4));
+    LN_CHECK_HIP(this, hipMalloc(&d_mapping, n * 16));
+    LN_CHECK_HIP(this, hipMalloc(&d_exponents, n * 4));
+    LN_CHECK_HIP(this, hipMalloc(&d_lyap, n * 4));
+    LN_CHECK_HIP(this, hipMalloc(&d_hit, n * 4));
+    LN_CHECK_HIP(this, hipMalloc(&d_steps, n * 4));
+    LN_CHECK_HIP(this, hipMemset(d_color, 0, n * 4));
+    LN_CHECK_HIP(this, hipMemset(d_final, 0, n * 4));
+    LN_CHECK_HIP(this, hipMemset(d_mapping, 0, n * 16));
+    LN_CHECK_HIP(this, hipMemset(d_exponents, 0, n * 4));
+    LN_CHECK_HIP(this, hipMemset(d_hit, 0xff, n * 4));
+    LN_CHECK_HIP(this, hipMemset(d_steps, 0, n * 4));
+    return LINON_OK;
+}
+
+linon_status Context::alloc_samples() {
+    const size_t bytes = (size_t)8 * std::max(1, integ.samples_per_ray) * 32;
+    if (d_samples && samples_bytes == bytes) return LINON_OK;
+    if (d_samples) (void)hipFree(d_samples);
+    d_samples = nullptr;
+    LN_CHECK_HIP(this, hipMalloc(&d_samples, bytes));
+    LN_CHECK_HIP(this, hipMemset(d_samples, 0, bytes));
+    samples_bytes = bytes;
+    return LINON_OK;
+}
+
+float Context::viewport_height() const {
+    // reference: fixed 0.025 at focal length 0.035 (src/main_view.wgsl:355-356)
+    if (camera.view_direction[3] > 0.0f)
+        return 2.0f * 0.035f * (float)std::tan((double)camera.view_direction[3] * 3.141592653589793 / 360.0);
+    return 0.025f;
+}
+
+void Context::fill_raycast_params(RaycastParams& P, const linon_integrator& ig) const {
+    std::memset(&P, 0, sizeof(P));
+    P.width = (int)width;
+    P.height = (int)height;
+    P.tiles_x = (int)((width + LINON_TILE - 1) / LINON_TILE);
+    P.tiles_y = (int)((height + LINON_TILE - 1) / LINON_TILE);
+    P.sample_block = (ig.samples_per_ray > 0 && ig.sample_steps > 0) ? 1 : 0;
+    for (int k = 0; k < 3; ++k) {
+        P.origin[k] = camera.origin[k];
+        P.view_dir[k] = camera.view_direction[k];
+        P.up[k] = camera.up[k];
+    }
+    P.viewport_h = viewport_height();
+    P.field_weight = settings.field_weight;
+    P.mouse_x = settings.mouse_pos[0];
+    P.mouse_y = settings.mouse_pos[1];
+    P.h_initial = ig.h_initial;
+    P.t_max = ig.t_max;
+    P.adaptive = ig.adaptive;
+    P.max_steps = ig.max_steps;
+    P.mode = ig.mode;
+    P.need_mapping = ig.need_mapping;
+    P.escape_r2 = ig.escape_radius * ig.escape_radius;
+    P.tris = d_tris;
+    P.num_faces = (int)num_faces;
+    P.accel = (accel_req == LINON_ACCEL_BRUTE) ? LINON_ACCEL_BRUTE_K : LINON_ACCEL_GRID_K;
+    P.cell_start = d_cell_start;
+    P.cell_faces = d_cell_faces;
+    P.dist = d_dist;
+    for (int k = 0; k < 3; ++k) {
+        P.grid_lo[k] = grid_lo[k];
+        P.grid_hi[k] = grid_hi[k];
+        P.gdim[k] = gdim[k];
+    }
+    P.cell_size = cell_size;
+    P.inv_cell_size = 1.0f / cell_size;
+    P.skip_margin = skip_margin;
+    P.color = d_color;
+    P.mapping = d_mapping;
+    P.hit_face = d_hit;
+    P.steps = d_steps;
+    P.counters = d_counters;
+    P.ray_samples = d_samples;
+    P.samples_per_ray = ig.samples_per_ray;
+    P.sample_steps = ig.sample_steps;
+}
+
+hipEvent_t* Context::next_timing_slot() {
+    if (ring.empty()) {
+        ring.assign(3 * LINON_TIMING_SLOTS, nullptr);
+        for (auto& e : ring)
+            if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+    }
+    hipEvent_t* slot = &ring[3 * (size_t)ring_next];
+    ring_next = (ring_next + 1) % LINON_TIMING_SLOTS;
+    ring_count = std::min(ring_count + 1, LINON_TIMING_SLOTS);
+    return slot;
+}
+
+linon_status Context::render_with(const linon_integrator& ig, hipStream_t s) {
+    if (!have_mesh) return fail(LINON_ERR_STATE, "render: no mesh set (linon_set_mesh)");
+    if (field_id < 0 && !jit_fn) return fail(LINON_ERR_STATE, "render: no field selected");
+    LN_CHECK_HIP(this, hipSetDevice(device));
+    if (!s) s = stream;
+    RaycastParams P;
+    fill_raycast_params(P, ig);
+    const unsigned blocks = (unsigned)(P.tiles_x * P.tiles_y + P.sample_block);
+    const size_t shmem = (P.accel == LINON_ACCEL_BRUTE_K && P.num_faces <= LINON_MAX_BRUTE_LDS)
+                             ? (size_t)P.num_faces * 48
+                             : 0;
+    LN_CHECK_HIP(this, launch_clear_u64(d_counters, LINON_NUM_COUNTERS, s));
+    hipEvent_t* slot = next_timing_slot();
+    for (int k = 0; k < 3; ++k) ev[k] = slot[k];
+    LN_CHECK_HIP(this, hipEventRecord(ev[0], s));
+    if (jit_fn) {
+        void* args[] = {&P};
+        LN_CHECK_HIP(this, hipModuleLaunchKernel(jit_fn, blocks, 1, 1, 64, 1, 1, (unsigned)shmem, s, args, nullptr));
+    } else {
+        LN_CHECK_HIP(this, launch_raycast_static(field_id, P, blocks, shmem, s));
+    }
+    LN_CHECK_HIP(this, hipEventRecord(ev[1], s));
+    OverlayParams O;
+    std::memset(&O, 0, sizeof(O));
+    O.width = (int)width;
+    O.height = (int)height;
+    O.overlay_mode = settings.overlay_mode;
+    O.delta = settings.central_difference_delta;
+    O.scaling = settings.lyapunov_scaling;
+    O.ray_casting = d_color;
+    O.mapping = d_mapping;
+    O.final_color = d_final;
+    O.exponents = d_exponents;
+    O.lyapunov = d_lyap;
+    LN_CHECK_HIP(this, launch_overlay(O, s));
+    LN_CHECK_HIP(this, hipEventRecord(ev[2], s));
+    timed = true;
+    return LINON_OK;
+}
+
+linon_status Context::read(const void* src, void* dst, size_t want, size_t bytes, const char* what) {
+    if (!dst) return fail(LINON_ERR_ARGUMENT, std::string(what) + ": null destination");
+    if (bytes < want)
+        return fail(LINON_ERR_ARGUMENT, std::string(what) + ": buffer too small (" + std::to_string(bytes) +
+                                            " < " + std::to_string(want) + ")");
+    LN_CHECK_HIP(this, hipSetDevice(device));
+    // renders may run on a caller's stream: wait for the whole device
+    LN_CHECK_HIP(this, hipDeviceSynchronize());
+    LN_CHECK_HIP(this, hipMemcpy(dst, src, want, hipMemcpyDeviceToHost));
+    return LINON_OK;
+}
+
+linon_status Context::upload_mesh(const float* xyz, uint32_t nv, const uint32_t* faces, uint32_t nf) {
+    for (size_t i = 0; i < 3 * (size_t)nf; ++i)
+        if (faces[i] >= nv) return fail(LINON_ERR_ARGUMENT, "set_mesh: face index out of range");
+    int res = grid_res_req;
+    if (res == 0) res = (nf <= 65536) ? 128 : 256;
+    GridBuild g;
+    std::string e;
+    if (!build_grid(xyz, nv, faces, nf, res, g, e)) return fail(LINON_ERR_ARGUMENT, "set_mesh: " + e);
+    std::vector<float> packed;
+    pack_faces(xyz, faces, nf, packed);
+    // keep host copies: a later linon_set_accel may rebuild the grid
+    if (h_xyz.data() != xyz) h_xyz.assign(xyz, xyz + 3 * (size_t)nv);
+    if (h_faces.data() != faces) h_faces.assign(faces, faces + 3 * (size_t)nf);
+    LN_CHECK_HIP(this, hipSetDevice(device));
+    LN_CHECK_HIP(this, hipDeviceSynchronize());
+    free_mesh();
+    const size_t tri_bytes = std::max<size_t>(packed.size() * 4, 48);
+    LN_CHECK_HIP(this, hipMalloc(&d_tris, tri_bytes));
+    if (!packed.empty()) LN_CHECK_HIP(this, hipMemcpy(d_tris, packed.data(), packed.size() * 4, hipMemcpyHostToDevice));
+    LN_CHECK_HIP(this, hipMalloc(&d_cell_start, g.cell_start.size() * 4));
+    LN_CHECK_HIP(this, hipMemcpy(d_cell_start, g.cell_start.data(), g.cell_start.size() * 4, hipMemcpyHostToDevice));
+    LN_CHECK_HIP(this, hipMalloc(&d_cell_faces, std::max<size_t>(g.cell_faces.size(), 1) * 4));
+    if (!g.cell_faces.empty())
+        LN_CHECK_HIP(this, hipMemcpy(d_cell_faces, g.cell_faces.data(), g.cell_faces.size() * 4, hipMemcpyHostToDevice));
+    LN_CHECK_HIP(this, hipMalloc(&d_dist, g.dist.size()));
+    LN_CHECK_HIP(this, hipMemcpy(d_dist, g.dist.data(), g.dist.size(), hipMemcpyHostToDevice));
+    for (int k = 0; k < 3; ++k) {
+        grid_lo[k] = g.lo[k];
+        grid_hi[k] = g.hi[k];
+        gdim[k] = g.dim[k];
+    }
+    cell_size = g.cell_size;
+    skip_margin = 4.0f * g.margin;
+    grid_refs = g.cell_faces.size();
+    num_faces = nf;
+    have_mesh = true;
+    return LINON_OK;
+}
+
+}  // namespace linon
+
+using linon::Context;
+
+#define LN_CTX(ctx)                         \
+    if (!(ctx)) return LINON_ERR_ARGUMENT;  \
+    (ctx)->err.clear();
+
+extern "C" {
+
+int linon_abi_version(void) { return LINON_ABI_VERSION; }
+
+linon_status linon_create(linon_context** out, int device, uint32_t width, uint32_t height) {
+    if (!out) return LINON_ERR_ARGUMENT;
+    *out = nullptr;
+    if (width < 2 || height < 2 || width > 16384 || height > 16384) return LINON_ERR_ARGUMENT;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return LINON_ERR_HIP;
+    Context* c = new (std::nothrow) Context();
+    if (!c) return LINON_ERR_HIP;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return LINON_ERR_HIP;
+    }
+    // MainView::new defaults (src/main_view.rs:100,133-139; src/main_view.wgsl:227,241-244,287-293)
+    c->settings = linon_settings{1.0f, {0.5f, 0.6f}, 0, 1, 50.0f};
+    c->integ = linon_integrator{0.1f, 5.0f, 1, 100000, LINON_MODE_REFERENCE, 2.0f, 100, 5000, 0, {0, 0, 0}};
+    c->field_id = LINON_FIELD_MIRAGE_SPHERICAL_SIGMOID;
+    // arcball default (src/arcball.rs:33-46 + zoom(-1)): eye = centre + (0,0,2) looking down -z
+    std::memset(&c->camera, 0, sizeof(c->camera));
+    c->camera.origin[2] = 2.0f;
+    c->camera.view_direction[2] = -1.0f;
+    c->camera.up[1] = 1.0f;
+    for (int i = 0; i < 4; ++i) c->camera.view_matrix[5 * i] = 1.0f;
+    if (hipMalloc(&c->d_counters, 8 * LINON_NUM_COUNTERS) != hipSuccess) {
+        delete c;
+        return LINON_ERR_HIP;
+    }
+    (void)hipMemset(c->d_counters, 0, 8 * LINON_NUM_COUNTERS);
+    if (c->alloc_images(width, height) != LINON_OK || c->alloc_samples() != LINON_OK) {
+        linon_destroy(reinterpret_cast<linon_context*>(c));
+        return LINON_ERR_HIP;
+    }
+    *out = reinterpret_cast<linon_context*>(c);
+    return LINON_OK;
+}
+
+void linon_destroy(linon_context* ctx) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->free_images();
+    c->free_mesh();
+    c->release_jit();
+    if (c->d_samples) (void)hipFree(c->d_samples);
+    if (c->d_counters) (void)hipFree(c->d_counters);
+    for (auto& e : c->ring)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* linon_last_error(const linon_context* ctx) {
+    const Context* c = reinterpret_cast<const Context*>(ctx);
+    if (!c) return "null context";
+    return c->err.c_str();
+}
+
+linon_status linon_resize(linon_context* ctx, uint32_t width, uint32_t height) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    if (width < 2 || height < 2 || width > 16384 || height > 16384)
+        return c->fail(LINON_ERR_ARGUMENT, "resize: width and height must be in [2, 16384]");
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    return c->alloc_images(width, height);
+}
+
+linon_status linon_set_camera(linon_context* ctx, const linon_camera* camera) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    if (!camera) return c->fail(LINON_ERR_ARGUMENT, "set_camera: null camera");
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(camera->origin[k]) || !std::isfinite(camera->view_direction[k]) ||
+            !std::isfinite(camera->up[k]))
+            return c->fail(LINON_ERR_ARGUMENT, "set_camera: non-finite vector");
+    if (!(camera->view_direction[3] >= 0.0f && camera->view_direction[3] < 180.0f))
+        return c->fail(LINON_ERR_ARGUMENT, "set_camera: view_direction.w (field of view) must be in [0, 180)");
+    c->camera = *camera;
+    return LINON_OK;
+}
+
+linon_status linon_set_settings(linon_context* ctx, const linon_settings* s) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    if (!s) return c->fail(LINON_ERR_ARGUMENT, "set_settings: null settings");
+    if (s->overlay_mode < 0 || s->overlay_mode > 1)
+        return c->fail(LINON_ERR_ARGUMENT, "set_settings: overlay_mode must be 0 or 1");
+    if (s->central_difference_delta < 1 || s->central_difference_delta > 1024)
+        return c->fail(LINON_ERR_ARGUMENT, "set_settings: central_difference_delta must be >= 1");
+    c->settings = *s;
+    return LINON_OK;
+}
+
+linon_status linon_set_integrator(linon_context* ctx, const linon_integrator* ig) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    if (!ig) return c->fail(LINON_ERR_ARGUMENT, "set_integrator: null");
+    if (!(ig->h_initial > 0.0f) || !std::isfinite(ig->h_initial) || !std::isfinite(ig->t_max))
+        return c->fail(LINON_ERR_ARGUMENT, "set_integrator: h_initial must be > 0 and t_max finite");
+    if (ig->max_steps < 0 || ig->max_steps > 100000000)
+        return c->fail(LINON_ERR_ARGUMENT, "set_integrator: max_steps must be in [0, 1e8]");
+    if (ig->mode != LINON_MODE_REFERENCE && ig->mode != LINON_MODE_ESCAPE)
+        return c->fail(LINON_ERR_ARGUMENT, "set_integrator: unknown mode");
+    if (ig->samples_per_ray < 0 || ig->samples_per_ray > 100000 || ig->sample_steps < 0 ||
+        (ig->samples_per_ray > 0 && ig->sample_steps < ig->samples_per_ray))
+        return c->fail(LINON_ERR_ARGUMENT, "set_integrator: need 0 <= samples_per_ray <= sample_steps");
+    c->integ = *ig;
+    return c->alloc_samples();
+}
+
+linon_status linon_get_integrator(const linon_context* ctx, linon_integrator* ig) {
+    const Context* c = reinterpret_cast<const Context*>(ctx);
+    if (!c || !ig) return LINON_ERR_ARGUMENT;
+    *ig = c->integ;
+    return LINON_OK;
+}
+
+linon_status linon_set_accel(linon_context* ctx, int accel, int grid_resolution) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    if (accel < LINON_ACCEL_AUTO || accel > LINON_ACCEL_GRID)
+        return c->fail(LINON_ERR_ARGUMENT, "set_accel: unknown accel");
+    if (grid_resolution != 0 && (grid_resolution < 4 || grid_resolution > 1024))
+        return c->fail(LINON_ERR_ARGUMENT, "set_accel: grid_resolution must be 0 (auto) or in [4, 1024]");
+    bool rebuild = c->have_mesh && grid_resolution != c->grid_res_req;
+    c->accel_req = accel;
+    c->grid_res_req = grid_resolution;
+    if (rebuild) return c->upload_mesh(c->h_xyz.data(), (uint32_t)(c->h_xyz.size() / 3), c->h_faces.data(),
+                                       (uint32_t)(c->h_faces.size() / 3));
+    return LINON_OK;
+}
+
+linon_status linon_set_mesh(linon_context* ctx, const float* xyz, uint32_t nv, const uint32_t* faces, uint32_t nf) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    if ((nv && !xyz) || (nf && !faces)) return c->fail(LINON_ERR_ARGUMENT, "set_mesh: null array");
+    return c->upload_mesh(xyz, nv, faces, nf);
+}
+
+linon_status linon_set_field_predefined(linon_context* ctx, int field) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    if (field == LINON_FIELD_CUSTOM)
+        return c->fail(LINON_ERR_ARGUMENT, "Custom has no predefined body; use linon_set_field_wgsl");
+    if (!linon::field_is_predefined(field)) return c->fail(LINON_ERR_ARGUMENT, "unknown predefined field id");
+    c->release_jit();
+    c->field_id = field;
+    return LINON_OK;
+}
+
+linon_status linon_render(linon_context* ctx, void* stream) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    return c->render_with(c->integ, reinterpret_cast<hipStream_t>(stream));
+}
+
+linon_status linon_render_high_accuracy(linon_context* ctx, void* stream) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    linon_integrator ig = c->integ;
+    ig.h_initial = 0.001f;  // src/main_view.rs:785-787
+    return c->render_with(ig, reinterpret_cast<hipStream_t>(stream));
+}
+
+linon_status linon_synchronize(linon_context* ctx) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    (void)hipSetDevice(c->device);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->fail(LINON_ERR_HIP, std::string("synchronize: ") + hipGetErrorString(e));
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) return c->fail(LINON_ERR_HIP, std::string("synchronize: ") + hipGetErrorString(e));
+    return LINON_OK;
+}
+
+linon_status linon_read_final(linon_context* ctx, uint8_t* rgba, size_t bytes) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    return c->read(c->d_final, rgba, (size_t)c->width * c->height * 4, bytes, "read_final");
+}
+linon_status linon_read_ray_casting(linon_context* ctx, uint8_t* rgba, size_t bytes) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    return c->read(c->d_color, rgba, (size_t)c->width * c->height * 4, bytes, "read_ray_casting");
+}
+linon_status linon_read_mapping(linon_context* ctx, float* xyzw, size_t bytes) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    return c->read(c->d_mapping, xyzw, (size_t)c->width * c->height * 16, bytes, "read_mapping");
+}
+linon_status linon_read_exponents(linon_context* ctx, float* data, size_t bytes) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    return c->read(c->d_exponents, data, (size_t)c->width * c->height * 4, bytes, "read_exponents");
+}
+linon_status linon_read_lyapunov(linon_context* ctx, float* data, size_t bytes) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    return c->read(c->d_lyap, data, (size_t)c->width * c->height * 4, bytes, "read_lyapunov");
+}
+linon_status linon_read_hit_faces(linon_context* ctx, int32_t* data, size_t bytes) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    return c->read(c->d_hit, data, (size_t)c->width * c->height * 4, bytes, "read_hit_faces");
+}
+linon_status linon_read_steps(linon_context* ctx, int32_t* data, size_t bytes) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    return c->read(c->d_steps, data, (size_t)c->width * c->height * 4, bytes, "read_steps");
+}
+linon_status linon_read_ray_samples(linon_context* ctx, float* data, size_t bytes) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    size_t want = (size_t)8 * std::max(1, c->integ.samples_per_ray) * 32;
+    return c->read(c->d_samples, data, want, bytes, "read_ray_samples");
+}
+
+linon_status linon_get_counters(linon_context* ctx, linon_counters* out) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    if (!out) return c->fail(LINON_ERR_ARGUMENT, "get_counters: null");
+    unsigned long long h[LINON_NUM_COUNTERS];
+    linon_status s = c->read(c->d_counters, h, sizeof(h), sizeof(h), "get_counters");
+    if (s != LINON_OK) return s;
+    std::memset(out, 0, sizeof(*out));
+    out->rk4_steps = h[LINON_CNT_STEPS];
+    out->triangle_tests = h[LINON_CNT_TESTS];
+    out->cell_visits = h[LINON_CNT_CELLS];
+    out->queries = h[LINON_CNT_QUERIES];
+    out->skips = h[LINON_CNT_SKIPS];
+    out->rays = h[LINON_CNT_RAYS];
+    if (c->timed && c->ev[0]) {
+        float a = 0, b = 0;
+        if (hipEventElapsedTime(&a, c->ev[0], c->ev[2]) == hipSuccess) out->last_render_ms = a;
+        if (hipEventElapsedTime(&b, c->ev[0], c->ev[1]) == hipSuccess) out->last_raycast_ms = b;
+    }
+    return LINON_OK;
+}
+
+linon_status linon_get_timings(linon_context* ctx, float* raycast_ms, float* render_ms, int capacity, int* count) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    if (!count || capacity < 0) return c->fail(LINON_ERR_ARGUMENT, "get_timings: null count");
+    if (hipSetDevice(c->device) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return c->fail(LINON_ERR_HIP, "get_timings: synchronize failed");
+    const int n = std::min(c->ring_count, capacity);
+    const int first = (c->ring_next - c->ring_count + LINON_TIMING_SLOTS) % LINON_TIMING_SLOTS;
+    for (int i = 0; i < n; ++i) {
+        hipEvent_t* e = &c->ring[3 * (size_t)((first + i) % LINON_TIMING_SLOTS)];
+        float a = -1.0f, b = -1.0f;
+        if (hipEventElapsedTime(&a, e[0], e[1]) != hipSuccess) a = -1.0f;
+        if (hipEventElapsedTime(&b, e[0], e[2]) != hipSuccess) b = -1.0f;
+        if (raycast_ms) raycast_ms[i] = a;
+        if (render_ms) render_ms[i] = b;
+    }
+    *count = n;
+    return LINON_OK;
+}
+
+linon_status linon_reset_timings(linon_context* ctx) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    c->ring_count = 0;
+    return LINON_OK;
+}
+
+linon_status linon_get_device_buffers(linon_context* ctx, linon_device_buffers* out) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    if (!out) return c->fail(LINON_ERR_ARGUMENT, "get_device_buffers: null");
+    out->final_rgba8 = c->d_final;
+    out->ray_casting_rgba8 = c->d_color;
+    out->mapping_rgba32f = c->d_mapping;
+    out->exponents_f32 = c->d_exponents;
+    out->lyapunov_f32 = c->d_lyap;
+    out->hit_face_i32 = c->d_hit;
+    out->steps_i32 = c->d_steps;
+    out->ray_samples = c->d_samples;
+    out->width = c->width;
+    out->height = c->height;
+    return LINON_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// Device-side state behind one linon_context (the compute pass of linon's MainView).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/linon.h"
+#include "device/params.hpp"
+
+namespace linon {
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    bool timed = false;
+    // per-render event triples (start, after ray cast, after overlay), a ring of slots
+    std::vector<hipEvent_t> ring;
+    int ring_next = 0, ring_count = 0;
+    std::string err;
+
+    uint32_t width = 0, height = 0;
+    linon_camera camera{};
+    linon_settings settings{};
+    linon_integrator integ{};
+
+    // field: a predefined id, or a kernel compiled at run time from a WGSL body
+    int field_id = -1;
+    hipModule_t jit_module = nullptr;
+    hipFunction_t jit_fn = nullptr;
+    std::string jit_body;
+
+    // images (row-major, texel row 0 at the top)
+    unsigned* d_color = nullptr;  // ray_casting_texture, rgba8
+    unsigned* d_final = nullptr;  // texture (overlay output), rgba8
+    float* d_mapping = nullptr;   // mapping_texture, rgba32f
+    float* d_exponents = nullptr; // exponents_buffer
+    float* d_lyap = nullptr;
+    int* d_hit = nullptr;
+    int* d_steps = nullptr;
+    float* d_samples = nullptr;
+    size_t samples_bytes = 0;
+    unsigned long long* d_counters = nullptr;
+
+    // mesh + grid
+    bool have_mesh = false;
+    uint32_t num_faces = 0;
+    float* d_tris = nullptr;
+    unsigned* d_cell_start = nullptr;
+    unsigned* d_cell_faces = nullptr;
+    unsigned char* d_dist = nullptr;
+    float grid_lo[3] = {0, 0, 0}, grid_hi[3] = {0, 0, 0};
+    int gdim[3] = {1, 1, 1};
+    float cell_size = 1.0f;
+    float skip_margin = 0.0f;
+    size_t grid_refs = 0;
+    int accel_req = LINON_ACCEL_AUTO;
+    int grid_res_req = 0;
+    std::vector<float> h_xyz;
+    std::vector<uint32_t> h_faces;
+
+    linon_status fail(linon_status s, const std::string& msg);
+    linon_status alloc_images(uint32_t w, uint32_t h);
+    linon_status alloc_samples();
+    void free_images();
+    void free_mesh();
+    void release_jit();
+    linon_status upload_mesh(const float* xyz, uint32_t nv, const uint32_t* faces, uint32_t nf);
+    float viewport_height() const;
+    void fill_raycast_params(RaycastParams& P, const linon_integrator& ig) const;
+    linon_status render_with(const linon_integrator& ig, hipStream_t s);
+    hipEvent_t* next_timing_slot();
+    linon_status read(const void* src, void* dst, size_t want, size_t bytes, const char* what);
+};
+
+}  // namespace linon

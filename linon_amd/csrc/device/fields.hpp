@@ -1,0 +1,107 @@
+// Predefined field functions as device functors.  Bodies 1..9 are the WGSL bodies of
+// PredefinedFunction::to_code (src/functions.rs:33-128) written in the lw:: vocabulary;
+// Identity is the stub of src/main_view.wgsl:148; Swirl/Lens/HotAir/Warp are the four
+// benchmark fields.  Every functor has the field_function signature
+//   (p_prev, p, v0, v, t) -> vec3   (src/main_view.wgsl:148, README.md:28-36).
+#pragma once
+#include "wgsl_lib.hpp"
+
+namespace lw {
+
+template <bool Spherical, bool Sigmoid>
+struct FieldMirage {  // src/functions.rs:36-96
+    __device__ __forceinline__ vec3 operator()(vec3 p_prev, vec3 p, vec3, vec3 v, float) const {
+        const float t_env = 15.0f;
+        const float t_src = Spherical ? 200.0f : 30.0f;
+        const float max_dist = Spherical ? 0.25f : 0.01f;
+        float dist_in, dist_out;
+        vec3 normal;
+        if (Spherical) {
+            const vec3 center = vec3f(-0.5f, 0.5f, -0.5f);
+            vec3 center_dest = p - center;
+            normal = normalize(center_dest);
+            dist_in = length(p_prev - center);
+            dist_out = length(center_dest);
+        } else {
+            const vec3 plane_p0 = vec3f(0.0f, 0.1f, 0.0f);
+            const vec3 plane_n = vec3f(0.0f, 1.0f, 0.0f);
+            normal = plane_n;
+            dist_in = point_plane_distance(p_prev, plane_n, plane_p0);
+            dist_out = point_plane_distance(p, plane_n, plane_p0);
+        }
+        float part_in, part_out;
+        if (Sigmoid) {
+            part_in = clamp(0.0f, 1.0f, sigmoid(dist_in / max_dist * 12.0f - 6.0f));
+            part_out = clamp(0.0f, 1.0f, sigmoid(dist_out / max_dist * 12.0f - 6.0f));
+        } else {
+            part_in = clamp(0.0f, 1.0f, dist_in / max_dist);
+            part_out = clamp(0.0f, 1.0f, dist_out / max_dist);
+        }
+        float t_in = part_in * t_env + (1.0f - part_in) * t_src;
+        float t_out = part_out * t_env + (1.0f - part_out) * t_src;
+        return refraction(t_in, t_out, v, normal);
+    }
+};
+using FieldMirageSpherical = FieldMirage<true, false>;
+using FieldMiragePlane = FieldMirage<false, false>;
+using FieldMirageSphericalSigmoid = FieldMirage<true, true>;
+using FieldMiragePlaneSigmoid = FieldMirage<false, true>;
+
+struct FieldTranslationX {  // src/functions.rs:98-101
+    __device__ __forceinline__ vec3 operator()(vec3, vec3, vec3 v0, vec3, float t) const {
+        return translate(v0, 0.5f * t, 0.0f, 0.0f);
+    }
+};
+struct FieldTranslationZ {  // src/functions.rs:103-106
+    __device__ __forceinline__ vec3 operator()(vec3, vec3, vec3 v0, vec3, float t) const {
+        return translate(v0, 0.0f, 0.0f, 0.5f * t);
+    }
+};
+struct FieldRotation {  // src/functions.rs:108
+    __device__ __forceinline__ vec3 operator()(vec3, vec3, vec3 v0, vec3, float t) const {
+        return normalize(v0 + rotateZ(v0, PI * t));
+    }
+};
+struct FieldLorenz {  // src/functions.rs:109-116
+    __device__ __forceinline__ vec3 operator()(vec3, vec3 p, vec3, vec3, float) const {
+        const float rho = 28.0f, sigma = 10.0f, beta = 8.0f / 3.0f;
+        return vec3f(sigma * (p.y - p.x), p.x * (rho - p.z) - p.y, p.x * p.y - beta * p.z);
+    }
+};
+struct FieldRoessler {  // src/functions.rs:118-125
+    __device__ __forceinline__ vec3 operator()(vec3, vec3 p, vec3, vec3, float) const {
+        const float a = 0.1f, b = 0.1f, c = 14.0f;
+        return vec3f(-p.y - p.z, p.x + a * p.y, b + p.z * (p.x - c));
+    }
+};
+struct FieldIdentity {  // src/main_view.wgsl:148
+    __device__ __forceinline__ vec3 operator()(vec3, vec3, vec3, vec3 v, float) const { return v; }
+};
+struct FieldSwirl {  // rotateY(v0, 0.5 * length(p))
+    __device__ __forceinline__ vec3 operator()(vec3, vec3 p, vec3 v0, vec3, float) const {
+        return rotateY(v0, 0.5f * length(p));
+    }
+};
+struct FieldLens {  // v - 0.3 * p / max(|p|^3, 1e-3)
+    __device__ __forceinline__ vec3 operator()(vec3, vec3 p, vec3, vec3 v, float) const {
+        float r = length(p);
+        return v - 0.3f * p / max(r * r * r, 0.001f);
+    }
+};
+struct FieldHotAir {  // refraction(T(p_prev), T(p), v, normalize(p - c)), T = 20 + 80 sigmoid(8 (0.5 - |p - c|))
+    __device__ __forceinline__ static float temperature(vec3 q) {
+        const vec3 c = vec3f(0.0f, -0.5f, 0.0f);
+        return 20.0f + 80.0f * sigmoid(8.0f * (0.5f - length(q - c)));
+    }
+    __device__ __forceinline__ vec3 operator()(vec3 p_prev, vec3 p, vec3, vec3 v, float) const {
+        const vec3 c = vec3f(0.0f, -0.5f, 0.0f);
+        return refraction(temperature(p_prev), temperature(p), v, normalize(p - c));
+    }
+};
+struct FieldWarp {  // v0 + 0.3 * (sin 4p.y, 0, sin 4p.x)
+    __device__ __forceinline__ vec3 operator()(vec3, vec3 p, vec3 v0, vec3, float) const {
+        return v0 + 0.3f * vec3f(sin(4.0f * p.y), 0.0f, sin(4.0f * p.x));
+    }
+};
+
+}  // namespace lw

@@ -1,0 +1,496 @@
+// The ray-cast kernel body, templated on the field functor.
+//
+// One lane traces one pixel's ray through the field with RK4 and intersects every step's
+// segment with the scene until the first hit (src/main_view.wgsl:229-279); one wave64 covers
+// an 8x8 pixel tile, like the reference's @workgroup_size(8, 8).  Block 0 of the launch can
+// trace the 8 sample rays of src/main_view.wgsl:377-480 so that this long serial work starts
+// first and overlaps the pixel tiles.
+//
+// Scene queries.  Brute force (LINON_ACCEL_BRUTE_K) tests all faces in index order as the
+// reference does, with up to LINON_MAX_BRUTE_LDS faces staged in LDS.  The grid path keeps a
+// per-lane lower bound on the distance to the mesh (from a Chebyshev distance field over a
+// uniform grid) and skips every segment that provably cannot reach a face; the remaining
+// segments walk the grid cells they cross (3-D DDA) and test the faces listed there.  Both
+// paths use the same Moeller-Trumbore arithmetic and the same (t, face index) lexicographic
+// minimum, so they return the same face and t bit for bit.
+#pragma once
+#include "params.hpp"
+#include "wgsl_lib.hpp"
+
+namespace lk {
+
+using lw::vec3;
+using lw::vec3f;
+using lw::dot;
+using lw::cross;
+using lw::length;
+using lw::normalize;
+
+__device__ __forceinline__ vec3 ld3(const float* p) { return vec3f(p[0], p[1], p[2]); }
+
+__device__ __forceinline__ vec3 ld_tri_vec(const float* tris, int face, int k) {
+    const float4 q = reinterpret_cast<const float4*>(tris)[3 * face + k];
+    return vec3f(q.x, q.y, q.z);
+}
+
+// Moeller-Trumbore in the operation order of hit_triangle (src/main_view.wgsl:150-175):
+// two-sided, |a| < 1e-7 rejected, inclusive edges, t > 1e-7 accepted.  e1 = v1 - v0 and
+// e2 = v2 - v0 are precomputed in f32, which gives the same bits as computing them here.
+__device__ __forceinline__ float mt_test(vec3 v0, vec3 e1, vec3 e2, vec3 o, vec3 d) {
+    const float eps = 0.0000001f;
+    vec3 h = cross(d, e2);
+    float a = dot(e1, h);
+    if (a > -eps && a < eps) return -1.0f;
+    float f = 1.0f / a;
+    vec3 s = o - v0;
+    float u = f * dot(s, h);
+    if (u < 0.0f || u > 1.0f) return -1.0f;
+    vec3 q = cross(s, e1);
+    float v = f * dot(d, q);
+    if (v < 0.0f || u + v > 1.0f) return -1.0f;
+    float t = f * dot(e2, q);
+    return t > eps ? t : -1.0f;
+}
+
+// Accept t_new when it is a valid hit on [0, max_dist) that precedes the best so far, with
+// the lower face index winning equal t (the reference's strict `<` over ascending faces).
+__device__ __forceinline__ void consider(float t_new, int face, float max_dist, float& best_t, int& best_face) {
+    if (t_new > 0.0f && t_new < max_dist &&
+        (best_t < 0.0f || t_new < best_t || (t_new == best_t && face < best_face))) {
+        best_t = t_new;
+        best_face = face;
+    }
+}
+
+struct QueryStats {
+    unsigned long long tests;
+    unsigned cells;
+    unsigned queries;
+    unsigned skips;
+};
+
+// Lower bound on the distance from p to any face: 0 inside an occupied cell; else the
+// distance to the boundary of the empty (2k-1)^3 block of cells around p's cell, where k is
+// the cell's Chebyshev distance to the nearest occupied cell; outside the grid, the distance
+// to the grid box (all faces lie inside it).
+__device__ __forceinline__ float grid_safe_distance(const RaycastParams& P, vec3 p) {
+    const float ics = P.inv_cell_size;
+    float gx = (p.x - P.grid_lo[0]) * ics;
+    float gy = (p.y - P.grid_lo[1]) * ics;
+    float gz = (p.z - P.grid_lo[2]) * ics;
+    const float nx = (float)P.gdim[0], ny = (float)P.gdim[1], nz = (float)P.gdim[2];
+    if (!(gx >= 0.0f && gx < nx && gy >= 0.0f && gy < ny && gz >= 0.0f && gz < nz)) {
+        float dx = fmaxf(fmaxf(-gx, gx - nx), 0.0f);
+        float dy = fmaxf(fmaxf(-gy, gy - ny), 0.0f);
+        float dz = fmaxf(fmaxf(-gz, gz - nz), 0.0f);
+        float d2 = dx * dx + dy * dy + dz * dz;
+        return (d2 > 0.0f) ? __builtin_sqrtf(d2) * P.cell_size : 0.0f;  // NaN -> 0 -> query
+    }
+    int ix = (int)gx, iy = (int)gy, iz = (int)gz;
+    int k = P.dist[((size_t)iz * P.gdim[1] + iy) * P.gdim[0] + ix];
+    if (k == 0) return 0.0f;
+    float lo = (float)(k - 1), hi = (float)k;
+    float bx = fminf(gx - ((float)ix - lo), ((float)ix + hi) - gx);
+    float by = fminf(gy - ((float)iy - lo), ((float)iy + hi) - gy);
+    float bz = fminf(gz - ((float)iz - lo), ((float)iz + hi) - gz);
+    return fminf(fminf(bx, by), bz) * P.cell_size;
+}
+
+__device__ __forceinline__ void test_cell(const RaycastParams& P, int cell, vec3 o, vec3 d, float L,
+                                          float& best_t, int& best_face, QueryStats& st) {
+    unsigned b = P.cell_start[cell], e = P.cell_start[cell + 1];
+    st.cells += 1;
+    st.tests += (e - b);
+    for (unsigned k = b; k < e; ++k) {
+        int f = (int)P.cell_faces[k];
+        float t = mt_test(ld_tri_vec(P.tris, f, 0), ld_tri_vec(P.tris, f, 1), ld_tri_vec(P.tris, f, 2), o, d);
+        consider(t, f, L, best_t, best_face);
+    }
+}
+
+// Walk the grid cells crossed by the segment o + s*d, s in [0, L] (3-D DDA).  The walk
+// takes exactly |dx|+|dy|+|dz| steps from the entry cell to the exit cell and only steps
+// along axes that still differ, so it always stays inside the grid and always ends; cells
+// grazed within rounding error are covered by the margin the cell lists are built with.
+__device__ __forceinline__ void query_grid(const RaycastParams& P, vec3 o, vec3 d, float L, float& best_t,
+                                           int& best_face, QueryStats& st) {
+    const float ics = P.inv_cell_size;
+    const float ax = (o.x - P.grid_lo[0]) * ics, ay = (o.y - P.grid_lo[1]) * ics, az = (o.z - P.grid_lo[2]) * ics;
+    const float sL = L * ics;
+    const float dx = d.x * sL, dy = d.y * sL, dz = d.z * sL;
+    const int nx = P.gdim[0], ny = P.gdim[1], nz = P.gdim[2];
+    float s0 = 0.0f, s1 = 1.0f;
+    // clip the segment to the grid box
+#define LK_CLIP(a, dd, n)                                              \
+    if (dd == 0.0f) {                                                  \
+        if (!(a >= 0.0f && a <= (float)n)) return;                     \
+    } else {                                                           \
+        float inv = 1.0f / dd;                                         \
+        float ta = (0.0f - a) * inv, tb = ((float)n - a) * inv;        \
+        s0 = fmaxf(s0, fminf(ta, tb));                                 \
+        s1 = fminf(s1, fmaxf(ta, tb));                                 \
+    }
+    LK_CLIP(ax, dx, nx)
+    LK_CLIP(ay, dy, ny)
+    LK_CLIP(az, dz, nz)
+#undef LK_CLIP
+    if (!(s0 <= s1)) return;
+    int cx = min(max((int)floorf(ax + dx * s0), 0), nx - 1);
+    int cy = min(max((int)floorf(ay + dy * s0), 0), ny - 1);
+    int cz = min(max((int)floorf(az + dz * s0), 0), nz - 1);
+    const int ex = min(max((int)floorf(ax + dx * s1), 0), nx - 1);
+    const int ey = min(max((int)floorf(ay + dy * s1), 0), ny - 1);
+    const int ez = min(max((int)floorf(az + dz * s1), 0), nz - 1);
+    const float INF = __builtin_inff();
+    const int sx = dx > 0.0f ? 1 : -1, sy = dy > 0.0f ? 1 : -1, sz = dz > 0.0f ? 1 : -1;
+    const float tdx = dx != 0.0f ? fabsf(1.0f / dx) : INF;
+    const float tdy = dy != 0.0f ? fabsf(1.0f / dy) : INF;
+    const float tdz = dz != 0.0f ? fabsf(1.0f / dz) : INF;
+    float tx = dx != 0.0f ? ((float)(dx > 0.0f ? cx + 1 : cx) - ax) / dx : INF;
+    float ty = dy != 0.0f ? ((float)(dy > 0.0f ? cy + 1 : cy) - ay) / dy : INF;
+    float tz = dz != 0.0f ? ((float)(dz > 0.0f ? cz + 1 : cz) - az) / dz : INF;
+    int remaining = abs(ex - cx) + abs(ey - cy) + abs(ez - cz);
+    const size_t nxy = (size_t)nx * ny;
+    while (true) {
+        test_cell(P, (int)(cz * nxy + (size_t)cy * nx + cx), o, d, L, best_t, best_face, st);
+        if (remaining <= 0) break;
+        --remaining;
+        float mx = (cx != ex) ? tx : INF;
+        float my = (cy != ey) ? ty : INF;
+        float mz = (cz != ez) ? tz : INF;
+        if (cx != ex && mx <= my && mx <= mz) {
+            cx += sx;
+            tx += tdx;
+        } else if (cy != ey && (my <= mz || cz == ez)) {
+            cy += sy;
+            ty += tdy;
+        } else if (cz != ez) {
+            cz += sz;
+            tz += tdz;
+        } else if (cx != ex) {
+            cx += sx;
+            tx += tdx;
+        } else {
+            cy += sy;
+            ty += tdy;
+        }
+    }
+}
+
+// (k1 + 2 k2 + 2 k3 + k4) / 6.0 with the correctly rounded quotient of the WGSL division:
+// q = s * RN(1/6), then one FMA residual correction (exact for every |s| above 2^-102), so
+// that e.g. a uniform field keeps V == v bit for bit as the f32 division does.
+__device__ __forceinline__ float div6(float s) {
+    const float inv6 = 1.0f / 6.0f;
+    float q = s * inv6;
+    float r = __builtin_fmaf(-q, 6.0f, s);
+    return __builtin_fmaf(r, inv6, q);
+}
+__device__ __forceinline__ vec3 rk4_average(vec3 k1, vec3 k2, vec3 k3, vec3 k4) {
+    vec3 s = ((k1 + 2.0f * k2) + 2.0f * k3) + k4;
+    return vec3f(div6(s.x), div6(s.y), div6(s.z));
+}
+
+__device__ __forceinline__ unsigned unorm8(float x) {
+    x = fminf(fmaxf(x, 0.0f), 1.0f);  // NaN -> 0
+    return (unsigned)__builtin_rintf(x * 255.0f);
+}
+__device__ __forceinline__ unsigned pack_rgba8(float r, float g, float b, float a) {
+    return unorm8(r) | (unorm8(g) << 8) | (unorm8(b) << 16) | (unorm8(a) << 24);
+}
+
+// Camera ray of invocation (gx, gy), src/main_view.wgsl:347-366.
+struct CameraFrame {
+    vec3 origin, vd, nh, nv;
+    float vw, vh;
+};
+__device__ __forceinline__ CameraFrame camera_frame(const RaycastParams& P) {
+    CameraFrame c;
+    c.origin = ld3(P.origin);
+    c.vd = ld3(P.view_dir);
+    vec3 up = ld3(P.up);
+    float width = (float)P.width, height = (float)P.height;
+    float aspect = width / height;
+    c.vh = P.viewport_h;
+    c.vw = aspect * c.vh;
+    vec3 w = -c.vd;
+    vec3 horizontal = cross(up, w);
+    vec3 vertical = cross(w, horizontal);
+    c.nh = normalize(horizontal);
+    c.nv = normalize(vertical);
+    return c;
+}
+__device__ __forceinline__ vec3 camera_dir(const CameraFrame& c, const RaycastParams& P, int gx, int gy) {
+    float width = (float)P.width, height = (float)P.height;
+    float u = (float)gx / (width - 1.0f) * c.vw - 0.5f * c.vw;
+    float v = (float)gy / (height - 1.0f) * c.vh - 0.5f * c.vh;
+    vec3 s = u * c.nh + v * c.nv + 0.035f * c.vd;
+    return normalize(s);
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// XCD-aware tile order: the dispatcher hands consecutive workgroups to the 8 XCDs in turn;
+// remap so that each XCD walks one contiguous run of tiles (neighbouring tiles touch the
+// same grid cells and faces, which then stay in that XCD's L2).
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    int q = n / LINON_NUM_XCD, r = n % LINON_NUM_XCD;
+    int x = b % LINON_NUM_XCD, i = b / LINON_NUM_XCD;
+    return x * q + min(x, r) + i;
+}
+
+// The 8 sample rays (src/main_view.wgsl:281-335,470-479): fixed h = 0.001, no scene queries,
+// the position before every stride-th step recorded as {(p, 1), (colour, 0.5)}.
+template <class F>
+__device__ void sample_rays(const RaycastParams& P, const CameraFrame& cam, int r) {
+    const F field{};
+    float ox, oy, cr, cg, cb;
+    switch (r) {
+        case 0: ox = -1; oy = -1; cr = 1.0f; cg = 0.0f; cb = 0.0f; break;
+        case 1: ox = 0;  oy = -1; cr = 0.5f; cg = 0.5f; cb = 0.0f; break;
+        case 2: ox = 1;  oy = -1; cr = 0.0f; cg = 1.0f; cb = 0.0f; break;
+        case 3: ox = 1;  oy = 0;  cr = 0.5f; cg = 1.0f; cb = 0.0f; break;
+        case 4: ox = 1;  oy = 1;  cr = 1.0f; cg = 1.0f; cb = 0.0f; break;
+        case 5: ox = 0;  oy = 1;  cr = 0.5f; cg = 0.5f; cb = 0.5f; break;
+        case 6: ox = -1; oy = 1;  cr = 0.0f; cg = 0.0f; cb = 1.0f; break;
+        default: ox = -1; oy = 0; cr = 0.5f; cg = 0.0f; cb = 0.5f; break;
+    }
+    float px = P.mouse_x + 0.01f * ox;
+    float py = P.mouse_y + 0.01f * oy;
+    float u2 = px * cam.vw - 0.5f * cam.vw;
+    float v2 = py * cam.vh - 0.5f * cam.vh;
+    vec3 s2 = u2 * cam.nh + v2 * cam.nv + 0.035f * cam.vd;
+    const vec3 start_dir = normalize(s2);
+    vec3 p = cam.origin, d = start_dir;
+    const float w = P.field_weight;
+    const float h = 0.001f, hh = 0.5f * h;
+    float t = 0.0f;
+    const int nsamp = P.samples_per_ray;
+    const int stride = P.sample_steps / nsamp;
+    float4* out = reinterpret_cast<float4*>(P.ray_samples);
+    for (int i = 0; i < P.sample_steps; ++i) {
+        vec3 k1 = field(p, p, start_dir, d, t);
+        vec3 k2 = field(p, p + hh * k1, start_dir, k1, t + hh);
+        vec3 k3 = field(p, p + hh * k2, start_dir, k2, t + hh);
+        vec3 k4 = field(p, p + h * k3, start_dir, k3, t + h);
+        vec3 V = rk4_average(k1, k2, k3, k4);
+        d = (1.0f - w) * d + w * V;
+        vec3 step = d * h;
+        if (i % stride == 0 && i / stride < nsamp) {
+            int idx = r * nsamp + i / stride;
+            out[2 * idx] = make_float4(p.x, p.y, p.z, 1.0f);
+            out[2 * idx + 1] = make_float4(cr, cg, cb, 0.5f);
+        }
+        p = p + step;
+        t = t + h;
+    }
+}
+
+template <class F>
+__device__ __forceinline__ void raycast_body(const RaycastParams& P) {
+    extern __shared__ float4 lds_tris[];
+    const F field{};
+    const CameraFrame cam = camera_frame(P);
+    int b = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (P.sample_block) {
+        if (b == 0) {
+            if (lane < 8 && P.ray_samples && P.samples_per_ray > 0) sample_rays<F>(P, cam, lane);
+            return;
+        }
+        b -= 1;
+    }
+    const int ntiles = P.tiles_x * P.tiles_y;
+    const int tile = xcd_tile(b, ntiles);
+    const int gx = (tile % P.tiles_x) * LINON_TILE + (lane & (LINON_TILE - 1));
+    const int gy = (tile / P.tiles_x) * LINON_TILE + (lane / LINON_TILE);
+
+    const bool brute = (P.accel == LINON_ACCEL_BRUTE_K);
+    const bool brute_lds = brute && P.num_faces <= LINON_MAX_BRUTE_LDS;
+    if (brute_lds) {  // stage the faces once per wave; every lane then reads them from LDS
+        const float4* src = reinterpret_cast<const float4*>(P.tris);
+        for (int i = lane; i < 3 * P.num_faces; i += blockDim.x) lds_tris[i] = src[i];
+        __syncthreads();
+    }
+    // lanes outside the image stay in the wave (zero work) so the wave reductions below see
+    // every lane
+    const bool active = gx < P.width && gy < P.height;
+
+    const vec3 start_dir = camera_dir(cam, P, gx, gy);
+    vec3 p = cam.origin;
+    vec3 d = start_dir;
+    const float w = P.field_weight;
+    const float h_initial = P.h_initial;
+    float h = h_initial;
+    float t = 0.0f;
+    vec3 last_v = vec3f(0.0f, 0.0f, 0.0f);
+    float last_diff = -1.0f;
+    bool has_color = false;
+    float cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    int face = -1;
+    float safe = 0.0f;  // grid path: lower bound on the distance from p to the mesh
+    QueryStats st{0ull, 0u, 0u, 0u};
+    int iters = 0;
+    const bool adaptive = P.adaptive != 0;
+    const bool escape = (P.mode == LINON_MODE_ESCAPE);
+    const float margin = P.skip_margin;
+
+    while (active && iters < P.max_steps) {
+        if (!escape && !(t <= P.t_max)) break;
+        ++iters;
+        const float hh = 0.5f * h;
+        vec3 k1 = field(p, p, start_dir, d, t);
+        vec3 k2 = field(p, p + hh * k1, start_dir, k1, t + hh);
+        vec3 k3 = field(p, p + hh * k2, start_dir, k2, t + hh);
+        vec3 k4 = field(p, p + h * k3, start_dir, k3, t + h);
+        vec3 V = rk4_average(k1, k2, k3, k4);
+        float diff = 0.0f;
+        if (adaptive) {  // src/main_view.wgsl:251-255
+            diff = length(V - last_v);
+            if (last_diff >= 0.0f && diff > 10.0f * last_diff && h > 0.002f) {
+                h = 0.001f;
+                continue;
+            }
+        }
+        d = (1.0f - w) * d + w * V;
+        const vec3 step = d * h;
+        if (!has_color) {
+            const float L = length(step);
+            bool query = (L > 0.0f && L < __builtin_inff());  // normalize(0) or NaN: no hit
+            if (query && !brute) {
+                if (!(safe > L + margin)) safe = grid_safe_distance(P, p);
+                query = !(safe > L + margin);
+                st.skips += query ? 0u : 1u;
+                safe -= L;
+            }
+            if (query) {
+                st.queries += 1;
+                const vec3 nd = step / L;  // normalize(step_dir)
+                float best_t = -1.0f;
+                int best_face = -1;
+                if (brute) {
+                    const int nf = P.num_faces;
+                    st.tests += (unsigned long long)nf;
+                    if (brute_lds) {
+                        for (int f = 0; f < nf; ++f) {
+                            float4 a = lds_tris[3 * f], e1 = lds_tris[3 * f + 1], e2 = lds_tris[3 * f + 2];
+                            float tt = mt_test(vec3f(a.x, a.y, a.z), vec3f(e1.x, e1.y, e1.z),
+                                               vec3f(e2.x, e2.y, e2.z), p, nd);
+                            consider(tt, f, L, best_t, best_face);
+                        }
+                    } else {
+                        for (int f = 0; f < nf; ++f) {
+                            float tt = mt_test(ld_tri_vec(P.tris, f, 0), ld_tri_vec(P.tris, f, 1),
+                                               ld_tri_vec(P.tris, f, 2), p, nd);
+                            consider(tt, f, L, best_t, best_face);
+                        }
+                    }
+                } else {
+                    query_grid(P, p, nd, L, best_t, best_face, st);
+                }
+                if (best_face >= 0) {  // shading of ray_color, src/main_view.wgsl:199-218
+                    has_color = true;
+                    face = best_face;
+                    vec3 e1 = ld_tri_vec(P.tris, best_face, 1), e2 = ld_tri_vec(P.tris, best_face, 2);
+                    vec3 n = normalize(cross(e1, e2));
+                    float intensity = fmaxf(dot(n, -nd), 0.0f);
+                    if (intensity == 0.0f) intensity = fmaxf(dot(-n, -nd), 0.0f);
+                    float spec = powf(intensity, 64.0f);
+                    float s = (0.01f + intensity) + spec;
+                    cr = s * fabsf(n.x);
+                    cg = s * fabsf(n.y);
+                    cb = s * fabsf(n.z);
+                }
+            }
+        }
+        p = p + step;
+        t = t + h;
+        if (adaptive) {  // src/main_view.wgsl:266-270
+            last_v = V;
+            if (2.0f * diff < last_diff && h < h_initial) h = h_initial;
+            last_diff = diff;
+        }
+        if (escape) {
+            if (has_color && !P.need_mapping) break;
+            if (dot(p, p) > P.escape_r2 && dot(p, d) > 0.0f) break;
+        }
+    }
+
+    if (active) {
+        const size_t texel = (size_t)(P.height - 1 - gy) * P.width + gx;
+        P.color[texel] = pack_rgba8(cr, cg, cb, 1.0f);
+        reinterpret_cast<float4*>(P.mapping)[texel] = make_float4(p.x, p.y, p.z, 1.0f);
+        if (P.hit_face) P.hit_face[texel] = face;
+        if (P.steps) P.steps[texel] = iters;
+    }
+
+    // per-wave counters, one atomic per counter per wave
+    unsigned long long c_steps = wave_sum((unsigned long long)iters);
+    unsigned long long c_tests = wave_sum(st.tests);
+    unsigned long long c_cells = wave_sum((unsigned long long)st.cells);
+    unsigned long long c_q = wave_sum((unsigned long long)st.queries);
+    unsigned long long c_rays = wave_sum(active ? 1ull : 0ull);
+    unsigned long long c_skip = wave_sum((unsigned long long)st.skips);
+    if (P.counters && lane == 0) {
+        atomicAdd(P.counters + LINON_CNT_STEPS, c_steps);
+        atomicAdd(P.counters + LINON_CNT_TESTS, c_tests);
+        atomicAdd(P.counters + LINON_CNT_CELLS, c_cells);
+        atomicAdd(P.counters + LINON_CNT_QUERIES, c_q);
+        atomicAdd(P.counters + LINON_CNT_RAYS, c_rays);
+        atomicAdd(P.counters + LINON_CNT_SKIPS, c_skip);
+    }
+}
+
+// src/overlay.wgsl:99-146, one lane per texel.
+__device__ __forceinline__ float lyapunov_exponent(const float4* m, int W, int x, int y, int delta) {
+    float4 xn = m[(size_t)y * W + x + delta], xp = m[(size_t)y * W + x - delta];
+    float4 yn = m[(size_t)(y + delta) * W + x], yp = m[(size_t)(y - delta) * W + x];
+    vec3 gx = 0.5f * (vec3f(xn.x, xn.y, xn.z) - vec3f(xp.x, xp.y, xp.z));
+    vec3 gy = 0.5f * (vec3f(yn.x, yn.y, yn.z) - vec3f(yp.x, yp.y, yp.z));
+    float m00 = dot(gx, gx), m01 = dot(gy, gx), m10 = dot(gx, gy), m11 = dot(gy, gy);
+    float a0 = m00 * m11 - m01 * m10;
+    float a1 = -(m00 + m11);
+    float e0, e1;
+    float discrim = a1 * a1 - 4.0f * a0;
+    if (discrim >= 0.0f) {
+        float root = __builtin_sqrtf(discrim);
+        e0 = (-a1 - root) * 0.5f;
+        e1 = (-a1 + root) * 0.5f;
+    } else {
+        float root = __builtin_sqrtf(-discrim);
+        e0 = -a1 * 0.5f;
+        e1 = root * 0.5f;
+    }
+    return __builtin_sqrtf(fmaxf(e0, e1));
+}
+
+__device__ __forceinline__ void overlay_body(const OverlayParams& P) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= P.width || y >= P.height) return;
+    const size_t idx = (size_t)y * P.width + x;
+    const unsigned c = P.ray_casting[idx];
+    unsigned outc = c;
+    float lam = -1.0f;
+    const int pad = 2 * P.delta;
+    if (P.overlay_mode == 1 && !(x < pad || y < pad || x >= P.width - pad || y >= P.height - pad)) {
+        lam = lyapunov_exponent(reinterpret_cast<const float4*>(P.mapping), P.width, x, y, P.delta);
+        float s = expf(P.scaling * lam - 5.0f);
+        size_t eidx = (size_t)P.height * y + x;  // row stride = height (src/overlay.wgsl:128)
+        if (eidx < (size_t)P.width * P.height) P.exponents[eidx] = s;
+        if (s >= 0.8f) {
+            float alpha = fminf(1.0f, s) * 0.5f;
+            float r = (float)(c & 255u) * (1.0f / 255.0f), g = (float)((c >> 8) & 255u) * (1.0f / 255.0f);
+            float b = (float)((c >> 16) & 255u) * (1.0f / 255.0f), a = (float)(c >> 24) * (1.0f / 255.0f);
+            outc = pack_rgba8(alpha + (1.0f - alpha) * r, alpha + (1.0f - alpha) * g,
+                              alpha + (1.0f - alpha) * b, alpha + (1.0f - alpha) * a);
+        }
+    }
+    P.final_color[idx] = outc;
+    if (P.lyapunov) P.lyapunov[idx] = lam;
+}
+
+}  // namespace lk

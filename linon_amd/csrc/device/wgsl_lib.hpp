@@ -1,0 +1,148 @@
+// Device-side WGSL vocabulary for field functions: vec3<f32> arithmetic, the WGSL built-ins
+// a field body may call, and the helper functions linon's shader defines for field bodies
+// (src/main_view.wgsl:76-146).  Predefined fields (fields.hpp) and bodies translated from
+// WGSL at run time (compiled by hiprtc) are both written against this header.
+#pragma once
+#ifndef __HIPCC_RTC__
+#include <hip/hip_runtime.h>
+#endif
+
+namespace lw {
+
+struct vec3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ vec3 vec3f(float x, float y, float z) { return vec3{x, y, z}; }
+__device__ __forceinline__ vec3 vec3f(float s) { return vec3{s, s, s}; }
+
+__device__ __forceinline__ vec3 operator+(vec3 a, vec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ vec3 operator-(vec3 a, vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ vec3 operator*(vec3 a, vec3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ vec3 operator/(vec3 a, vec3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+__device__ __forceinline__ vec3 operator-(vec3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ vec3 operator*(float s, vec3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ vec3 operator*(vec3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ vec3 operator/(vec3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ vec3 operator/(float s, vec3 a) { return {s / a.x, s / a.y, s / a.z}; }
+__device__ __forceinline__ vec3 operator+(vec3 a, float s) { return {a.x + s, a.y + s, a.z + s}; }
+__device__ __forceinline__ vec3 operator+(float s, vec3 a) { return {s + a.x, s + a.y, s + a.z}; }
+__device__ __forceinline__ vec3 operator-(vec3 a, float s) { return {a.x - s, a.y - s, a.z - s}; }
+__device__ __forceinline__ vec3 operator-(float s, vec3 a) { return {s - a.x, s - a.y, s - a.z}; }
+
+__device__ __forceinline__ float dot(vec3 a, vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ vec3 cross(vec3 a, vec3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(vec3 a) { return __builtin_sqrtf(dot(a, a)); }
+__device__ __forceinline__ float length(float a) { return __builtin_fabsf(a); }
+__device__ __forceinline__ vec3 normalize(vec3 a) { return a / length(a); }
+__device__ __forceinline__ float distance(vec3 a, vec3 b) { return length(a - b); }
+__device__ __forceinline__ vec3 reflect(vec3 e1, vec3 e2) { return e1 - (2.0f * dot(e2, e1)) * e2; }
+
+// Scalar built-ins (f32) and their component-wise vec3 forms.
+#define LW_UNARY(name, expr)                                                        \
+    __device__ __forceinline__ float name(float x) { return expr; }                \
+    __device__ __forceinline__ vec3 name(vec3 v) { return {name(v.x), name(v.y), name(v.z)}; }
+LW_UNARY(sin, ::sinf(x))
+LW_UNARY(cos, ::cosf(x))
+LW_UNARY(tan, ::tanf(x))
+LW_UNARY(asin, ::asinf(x))
+LW_UNARY(acos, ::acosf(x))
+LW_UNARY(atan, ::atanf(x))
+LW_UNARY(sinh, ::sinhf(x))
+LW_UNARY(cosh, ::coshf(x))
+LW_UNARY(tanh, ::tanhf(x))
+LW_UNARY(exp, ::expf(x))
+LW_UNARY(exp2, ::exp2f(x))
+LW_UNARY(log, ::logf(x))
+LW_UNARY(log2, ::log2f(x))
+LW_UNARY(sqrt, __builtin_sqrtf(x))
+LW_UNARY(inverseSqrt, 1.0f / __builtin_sqrtf(x))
+LW_UNARY(abs, __builtin_fabsf(x))
+LW_UNARY(floor, __builtin_floorf(x))
+LW_UNARY(ceil, __builtin_ceilf(x))
+LW_UNARY(round, __builtin_rintf(x))
+LW_UNARY(trunc, __builtin_truncf(x))
+LW_UNARY(fract, x - __builtin_floorf(x))
+LW_UNARY(sign, (x > 0.0f) ? 1.0f : ((x < 0.0f) ? -1.0f : 0.0f))
+LW_UNARY(saturate, __builtin_fminf(__builtin_fmaxf(x, 0.0f), 1.0f))
+LW_UNARY(radians, x * 0.017453292519943295f)
+LW_UNARY(degrees, x * 57.29577951308232f)
+#undef LW_UNARY
+
+__device__ __forceinline__ float atan2(float y, float x) { return ::atan2f(y, x); }
+__device__ __forceinline__ float pow(float x, float y) { return ::powf(x, y); }
+__device__ __forceinline__ vec3 pow(vec3 x, vec3 y) { return {pow(x.x, y.x), pow(x.y, y.y), pow(x.z, y.z)}; }
+__device__ __forceinline__ float min(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ float max(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ vec3 min(vec3 a, vec3 b) { return {min(a.x, b.x), min(a.y, b.y), min(a.z, b.z)}; }
+__device__ __forceinline__ vec3 max(vec3 a, vec3 b) { return {max(a.x, b.x), max(a.y, b.y), max(a.z, b.z)}; }
+__device__ __forceinline__ int min(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int max(int a, int b) { return a > b ? a : b; }
+// WGSL clamp(e, low, high) = min(max(e, low), high); linon calls it as clamp(0.0, 1.0, x)
+// (src/functions.rs:45-46) which therefore evaluates to min(1, x).
+__device__ __forceinline__ float clamp(float e, float lo, float hi) { return min(max(e, lo), hi); }
+__device__ __forceinline__ vec3 clamp(vec3 e, vec3 lo, vec3 hi) { return min(max(e, lo), hi); }
+__device__ __forceinline__ float mix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ vec3 mix(vec3 a, vec3 b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ float step(float edge, float x) { return x < edge ? 0.0f : 1.0f; }
+__device__ __forceinline__ float smoothstep(float e0, float e1, float x) {
+    float t = clamp((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+__device__ __forceinline__ float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// ---- helpers of src/main_view.wgsl:76-146 ----------------------------------------------
+constexpr float PI = 3.141592653589793f;
+
+// The three rotations multiply by mat3x3 columns as the WGSL text lays them out; rotateZ's
+// third column is zero, so its z component is always 0 (src/main_view.wgsl:94-100).
+__device__ __forceinline__ vec3 rotateX(vec3 v, float phi) {
+    float s, c;
+    ::sincosf(phi, &s, &c);
+    return {v.x, c * v.y - s * v.z, s * v.y + c * v.z};
+}
+__device__ __forceinline__ vec3 rotateY(vec3 v, float phi) {
+    float s, c;
+    ::sincosf(phi, &s, &c);
+    return {c * v.x + s * v.z, v.y, -s * v.x + c * v.z};
+}
+__device__ __forceinline__ vec3 rotateZ(vec3 v, float phi) {
+    float s, c;
+    ::sincosf(phi, &s, &c);
+    return {c * v.x - s * v.y, s * v.x + c * v.y, 0.0f};
+}
+__device__ __forceinline__ vec3 translate(vec3 v, float dx, float dy, float dz) {
+    return {v.x + dx, v.y + dy, v.z + dz};
+}
+// Air refraction index at temperature t (deg C), Zhao et al.; pow(10.0, -10.0) is a
+// constant expression and enters as 1e-10.
+__device__ __forceinline__ float refraction_index(float t) {
+    const float air_pressure = 101325.0f;
+    const float c1 = 0.0000104f;
+    const float c2 = 0.00366f;
+    return c1 * air_pressure * (1.0f + air_pressure * (60.1f - 0.972f * t) * 1e-10f) / (1.0f + c2 * t);
+}
+__device__ __forceinline__ vec3 refraction(float t_in, float t_out, vec3 v_in, vec3 n) {
+    float eta_in = refraction_index(t_in);
+    float eta_out = refraction_index(t_out);
+    float cosi = clamp(-1.0f, 1.0f, dot(v_in, n));
+    vec3 n_ref = n;
+    if (cosi < 0.0f) {
+        cosi = -cosi;
+    } else {
+        n_ref = -n;
+    }
+    float eta = eta_in / eta_out;
+    float k = 1.0f - eta * eta * (1.0f - cosi * cosi);
+    if (k < 0.0f) return reflect(v_in, n_ref);  // total internal reflection
+    return eta * v_in + (eta * cosi - __builtin_sqrtf(k)) * n_ref;
+}
+__device__ __forceinline__ float point_plane_distance(vec3 p, vec3 n, vec3 p0) {
+    float d = dot(p0, n);
+    return __builtin_fabsf(dot(p, n) - d);
+}
+__device__ __forceinline__ float sigmoid(float x) { return 1.0f / (1.0f + ::expf(-x)); }
+
+}  // namespace lw

@@ -1,0 +1,220 @@
+"""GPU parity: the HIP ray-cast and overlay kernels (through the C ABI) against the FP32 CPU
+oracle on the same camera, field, step size and mesh."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import linon_amd as L
+from linon_amd import configs, scenes
+from oracle import oracle as O
+
+from gpu_common import EXIT_RTOL, HIT_MIN_AGREE, compare, rel_err, run_gpu, run_oracle, run_oracles
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def view():
+    v = L.MainView(64, 48)
+    yield v
+    v.close()
+
+
+def _case(view, W, H, cam, st, ig, field, verts, faces, label, accel=L.ACCEL_AUTO):
+    print(f"[case] {label} {W}x{H} field={field} accel={accel}", flush=True)
+    view.resize_texture(W, H)
+    gpu = run_gpu(view, cam, st, ig, field, verts, faces, accel)
+    fid = L.FIELDS[field] if isinstance(field, str) else field
+    ref, ref64 = run_oracles(cam, st, ig, fid, verts, faces, W, H)
+    info = compare(gpu, ref, ref64, label)
+    assert info["low_lambda"] > 0, info
+    assert info["hit_ok"], info
+    assert info["exit_ok"] >= HIT_MIN_AGREE, info
+    assert info["color_maxdiff"] <= 1, info
+    # rays whose adaptive steps were chosen by rounding noise still find the same faces
+    assert info["noisy_hit_agree"] >= 0.95, info
+    return gpu, ref
+
+
+def test_reference_default_scene(view):
+    """linon's start-up state: Cornell box (x, z negated, normalised), arcball default camera,
+    MirageSphericalSigmoid, h 0.1 adaptive, t <= 5 (src/application.rs:105-176)."""
+    v, f, c = scenes.reference_startup_scene()
+    gpu, ref = _case(view, 64, 48, L.Camera.arcball_default(c), L.Settings.make(), L.Integrator.make(),
+                     "mirage_spherical_sigmoid", v, f, "reference-default")
+    assert (ref["hit"] >= 0).mean() > 0.5
+    quiet = ref["noise_flag"] == 0
+    assert np.array_equal(gpu.steps[quiet], ref["steps"][quiet])
+
+
+@pytest.mark.parametrize("field", ["identity", "swirl", "lens", "hot_air", "warp"])
+def test_config_camera_cornell(view, field):
+    wl = configs.CONFIG1
+    v, f = wl.mesh()
+    _case(view, 64, 64, wl.camera(), wl.settings(), wl.integrator(), field, v, f, f"cornell-{field}")
+
+
+@pytest.mark.parametrize("field", ["swirl", "lens", "hot_air", "warp"])
+def test_benchmark_scene_small(view, field):
+    wl = configs.CONFIG2
+    v, f = scenes.benchmark_scene(level=2)
+    _case(view, 48, 32, wl.camera(), wl.settings(), wl.integrator(), field, v, f, f"bench-scene-{field}")
+
+
+@pytest.mark.parametrize("fid", [1, 2, 3, 4, 5, 6, 7, 8, 9])
+def test_predefined_fields(view, fid):
+    """Every body of PredefinedFunction (src/functions.rs:33-128) on the start-up scene."""
+    v, f, c = scenes.reference_startup_scene()
+    ig = L.Integrator.make(h_initial=0.05, samples_per_ray=0, sample_steps=0)
+    _case(view, 64, 48, L.Camera.arcball_default(c), L.Settings.make(), ig, fid, v, f, f"predefined-{fid}")
+
+
+def test_grid_equals_brute_force(view):
+    """The grid path (distance-field skip + cell walk) returns the same face, colour and end
+    point as the reference's loop over all faces, bit for bit."""
+    wl = configs.CONFIG2
+    v, f = scenes.benchmark_scene(level=3)
+    view.resize_texture(96, 64)
+    outs = []
+    for accel in (L.ACCEL_BRUTE, L.ACCEL_GRID):
+        print(f"[case] grid-vs-brute accel={accel}", flush=True)
+        fr = run_gpu(view, wl.camera(), wl.settings(), wl.integrator(), "warp", v, f, accel)
+        outs.append((fr, view.counters()))
+    (b, cb), (g, cg) = outs
+    assert (b.hit_faces >= 0).mean() > 0.03
+    assert np.array_equal(b.hit_faces, g.hit_faces)
+    assert np.array_equal(b.ray_casting, g.ray_casting)
+    assert np.array_equal(b.mapping, g.mapping)
+    assert cg["triangle_tests"] < cb["triangle_tests"]
+
+
+def test_overlay_kernel(view):
+    """Overlay kernel against the oracle's overlay applied to the GPU's own ray-cast outputs."""
+    v, f, c = scenes.reference_startup_scene()
+    W, H = 80, 60
+    view.resize_texture(W, H)
+    for delta in (1, 3):
+        st = L.Settings.make(overlay_mode=1, central_difference_delta=delta, lyapunov_scaling=50.0)
+        gpu = run_gpu(view, L.Camera.arcball_default(c), st, L.Integrator.make(), "mirage_spherical_sigmoid", v, f)
+        ov = O.overlay(O.as_oracle_struct(st, O.SettingsIn), gpu.ray_casting, gpu.mapping)
+        assert np.abs(gpu.final.astype(int) - ov["final"].astype(int)).max() <= 1
+        inner = ov["lyapunov"] >= 0
+        assert np.array_equal(inner, gpu.lyapunov >= 0)
+        # lambda via tr^2 - 4 det cancels when the two singular values are close; hold the GPU
+        # to the FP32 restatement's own error against FP64 on the same inputs
+        ov64 = O.overlay(O.as_oracle_struct(st, O.SettingsIn), gpu.ray_casting, gpu.mapping.astype(np.float64),
+                         precision="f64")
+        l64 = ov64["lyapunov"][inner].astype(np.float64)
+        e32 = np.abs(ov["lyapunov"][inner] - l64)
+        eg = np.abs(gpu.lyapunov[inner] - l64)
+        assert (eg <= np.maximum(4 * e32, 2e-5 * np.abs(l64) + 1e-7)).mean() >= 0.999
+        # exponents buffer: row stride H, compare the slots only one texel writes
+        idx = np.arange(W * H).reshape(H, W)
+        ey, ex = np.nonzero(inner)
+        slots = H * ey + ex
+        ok = slots < W * H
+        uniq, counts = np.unique(slots[ok], return_counts=True)
+        single = np.isin(slots, uniq[counts == 1]) & ok
+        e_gpu = gpu.exponents[slots[single]]
+        e_ref = ov["exponents"][slots[single]]
+        np.testing.assert_allclose(e_gpu, e_ref, rtol=1e-3, atol=1e-6)
+        del idx
+
+
+def test_overlay_disabled_copies_colour(view):
+    v, f, c = scenes.reference_startup_scene()
+    view.resize_texture(40, 30)
+    gpu = run_gpu(view, L.Camera.arcball_default(c), L.Settings.make(overlay_mode=0), L.Integrator.make(),
+                  "mirage_spherical_sigmoid", v, f)
+    assert np.array_equal(gpu.final, gpu.ray_casting)
+
+
+def test_sample_rays(view):
+    """The 8 sample rays (src/main_view.wgsl:281-314,470-479) against the FP32 oracle."""
+    v, f, c = scenes.reference_startup_scene()
+    cam = L.Camera.arcball_default(c)
+    st = L.Settings.make(mouse_pos=(0.4, 0.55))
+    ig = L.Integrator.make()
+    view.resize_texture(64, 48)
+    run_gpu(view, cam, st, ig, "mirage_spherical_sigmoid", v, f)
+    got = view.read_ray_samples()
+    c2, s2, i2 = (O.as_oracle_struct(cam, O.CameraIn), O.as_oracle_struct(st, O.SettingsIn),
+                  O.as_oracle_struct(ig, O.IntegratorIn))
+    ref = O.sample_rays(c2, s2, i2, 3, 64, 48)
+    assert got.shape == ref.shape
+    np.testing.assert_array_equal(got[:, :, 1], ref[:, :, 1])   # colours
+    pos_err = np.abs(got[:, :, 0, :3] - ref[:, :, 0, :3]) / np.maximum(1.0, np.abs(ref[:, :, 0, :3]))
+    assert pos_err.max() <= 1e-3, pos_err.max()
+
+
+def test_high_accuracy_render(view):
+    """Enhance (src/main_view.rs:778-802): one render at h_initial = 0.001."""
+    v, f, c = scenes.reference_startup_scene()
+    cam = L.Camera.arcball_default(c)
+    W, H = 24, 16
+    view.resize_texture(W, H)
+    run_gpu(view, cam, L.Settings.make(), L.Integrator.make(samples_per_ray=0, sample_steps=0),
+            "mirage_spherical_sigmoid", v, f)
+    view.render_high_accuracy()
+    gpu = view.read_frame()
+    assert view.integrator().h_initial == pytest.approx(0.1)
+    ref, ref64 = run_oracles(cam, L.Settings.make(),
+                             L.Integrator.make(h_initial=0.001, samples_per_ray=0, sample_steps=0), 3, v, f, W, H)
+    info = compare(gpu, ref, ref64, "enhance")
+    assert info["gated"] > 0 and info["hit_ok"] and info["exit_ok"] >= HIT_MIN_AGREE
+    assert int(np.median(gpu.steps)) >= 5000
+
+
+def test_counters_and_errors(view):
+    v, f = scenes.cornell_box_centered()
+    wl = configs.CONFIG1
+    view.resize_texture(32, 32)
+    run_gpu(view, wl.camera(), wl.settings(), wl.integrator(), "identity", v, f)
+    c = view.counters()
+    assert c["rays"] == 32 * 32
+    assert c["rk4_steps"] == int(view.read_steps().sum())
+    assert c["last_raycast_ms"] > 0
+    with pytest.raises(L.LinonError):
+        view.set_field(0)          # Custom has no predefined body
+    with pytest.raises(L.LinonError):
+        view.update_model(v, np.array([[0, 1, 999]], np.uint32))
+    with pytest.raises(L.LinonError):
+        view.update_settings(L.Settings.make(overlay_mode=7))
+
+
+@pytest.mark.parametrize("field", ["swirl", "lens", "hot_air", "warp"])
+def test_config2_full_resolution_sample(field):
+    """The 1920x1080 benchmark frame (Cornell box + 20,480-face icosphere, h 0.005): the whole
+    frame on the GPU, 1024 random pixels through the FP32 and FP64 oracles; lambda from the
+    GPU's own overlay at full resolution."""
+    wl = configs.CONFIG2
+    v, f = wl.mesh()
+    with L.MainView(wl.width, wl.height) as big:
+        big.update_model(v, f)
+        configs.setup_view(big, wl, field)
+        big.render()
+        fr = big.read_frame()
+    rng = np.random.default_rng(7)
+    gy = rng.integers(0, wl.height, 1024)
+    gx = rng.integers(0, wl.width, 1024)
+    px = np.stack([gx, gy], 1)
+    ty = wl.height - 1 - gy  # texel row of invocation row gy
+    c, s, i = (O.as_oracle_struct(wl.camera(), O.CameraIn), O.as_oracle_struct(wl.settings(), O.SettingsIn),
+               O.as_oracle_struct(wl.integrator(), O.IntegratorIn))
+    r32 = O.render_pixels(c, s, i, L.FIELDS[field], v, f, wl.width, wl.height, px, 16)
+    m64 = O.render_pixels(c, s, i, L.FIELDS[field], v, f, wl.width, wl.height, px, 16, precision="f64")["mapping"]
+    class G:
+        pass
+    g = G()
+    g.hit_faces = fr.hit_faces[ty, gx][None]
+    g.mapping = fr.mapping[ty, gx][None]
+    g.ray_casting = fr.ray_casting[ty, gx][None]
+    ref = {"mapping": r32["mapping"][None], "hit": r32["hit"][None], "noise_flag": np.zeros((1, len(px)), np.uint8),
+           "color8": g.ray_casting}
+    ref64 = {"mapping": m64[None]}
+    lam = fr.lyapunov[ty, gx][None]
+    info = compare(g, ref, ref64, f"config2-{field}", lam_gpu=lam)
+    assert info["gated"] > 100, info
+    assert info["hit_ok"], info
+    assert info["exit_ok"] >= HIT_MIN_AGREE, info

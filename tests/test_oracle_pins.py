@@ -1,0 +1,372 @@
+"""Pins of the CPU oracle against what the mathematics fixes (no GPU needed).
+
+Each test is chosen so that a plausible mistake in the restatement (a dropped term, a wrong
+sign or index, a swapped operand, a wrong stage weight) fails it.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from linon_amd import FIELDS, Camera, scenes
+from oracle import oracle as O
+
+
+# ---------------------------------------------------------------------------------------
+# Moeller-Trumbore (src/main_view.wgsl:150-175)
+# ---------------------------------------------------------------------------------------
+TRI = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0], np.float64)
+
+
+@pytest.mark.parametrize("o,d,t", [
+    ((0.25, 0.25, 1.0), (0, 0, -1), 1.0),     # interior
+    ((0.25, 0.25, -2.0), (0, 0, 1), 2.0),     # two-sided: from behind
+    ((0.5, 0.0, 1.0), (0, 0, -1), 1.0),       # on edge v = 0 (inclusive)
+    ((0.0, 0.5, 1.0), (0, 0, -1), 1.0),       # on edge u = 0
+    ((0.5, 0.5, 1.0), (0, 0, -1), 1.0),       # on edge u + v = 1
+    ((1.0, 0.0, 1.0), (0, 0, -1), 1.0),       # vertex
+    ((0.6, 0.6, 1.0), (0, 0, -1), -1.0),      # outside u + v > 1
+    ((-0.1, 0.2, 1.0), (0, 0, -1), -1.0),     # outside u < 0
+    ((0.25, 0.25, 1.0), (1, 0, 0), -1.0),     # parallel
+    ((0.25, 0.25, 1.0), (0, 0, 1), -1.0),     # behind the origin
+    ((0.25, 0.25, 0.0), (0, 0, -1), -1.0),    # t = 0 is not > eps
+])
+def test_mt_known_answers(o, d, t):
+    assert O.hit_triangle(TRI, o, d) == pytest.approx(t, abs=1e-12)
+    assert O.hit_triangle(TRI, o, d, precision="f32") == pytest.approx(t, abs=1e-6)
+
+
+def test_mt_degenerate_determinant():
+    tiny = TRI * 1e-4  # edge lengths 1e-4: a = 1e-8 < 1e-7 is rejected
+    assert O.hit_triangle(tiny, (2e-5, 2e-5, 1.0), (0, 0, -1)) == -1.0
+    small = TRI * 1e-3  # a = 1e-6 is accepted
+    assert O.hit_triangle(small, (2e-4, 2e-4, 1.0), (0, 0, -1)) == pytest.approx(1.0)
+
+
+def test_mt_random_against_plane_barycentric():
+    """FP64 oracle against an independent plane-intersection + barycentric solve."""
+    rng = np.random.default_rng(0)
+    n_checked = 0
+    for _ in range(2000):
+        v = rng.normal(size=(3, 3))
+        o = rng.normal(size=3) * 2
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        t = O.hit_triangle(v.ravel(), o, d)
+        # solve o + t d = v0 + u e1 + w e2
+        A = np.stack([-d, v[1] - v[0], v[2] - v[0]], 1)
+        if abs(np.linalg.det(A)) < 1e-6:
+            continue
+        tt, u, w = np.linalg.solve(A, o - v[0])
+        inside = u >= 0 and w >= 0 and u + w <= 1 and tt > 1e-7
+        margin = min(abs(u), abs(w), abs(1 - u - w), abs(tt))
+        if margin < 1e-9:
+            continue
+        n_checked += 1
+        if inside:
+            assert t == pytest.approx(tt, rel=1e-9, abs=1e-9)
+        else:
+            assert t == -1.0
+    assert n_checked > 1500
+
+
+def test_ray_color_closest_and_ties():
+    # two coplanar copies of one triangle (faces 0 and 2) and a nearer one (face 1)
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 0.5], [1, 0, 0.5], [0, 1, 0.5]], np.float32)
+    f = np.array([[0, 1, 2], [3, 4, 5], [0, 1, 2]], np.uint32)
+    face, rgba = O.ray_color(v, f, (0.2, 0.2, 1.0), (0, 0, -1), 10.0)
+    assert face == 1 and rgba[3] == pytest.approx(0.5)
+    # lowest index wins an exact tie
+    face, _ = O.ray_color(v, np.array([[3, 4, 5], [0, 1, 2], [3, 4, 5]], np.uint32), (0.2, 0.2, 1.0), (0, 0, -1), 10)
+    assert face == 0
+    f2 = np.array([[0, 1, 2], [0, 1, 2]], np.uint32)
+    face, _ = O.ray_color(v, f2, (0.2, 0.2, 1.0), (0, 0, -1), 10.0)
+    assert face == 0
+    # max_dist is exclusive: t must be < max_dist
+    face, _ = O.ray_color(v, f2, (0.2, 0.2, 1.0), (0, 0, -1), 1.0)
+    assert face == -1
+    face, _ = O.ray_color(v, f2, (0.2, 0.2, 1.0), (0, 0, -1), 1.0001)
+    assert face == 0
+
+
+def test_ray_color_shading_closed_form():
+    """Camera-as-light Blinn-Phong (src/main_view.wgsl:199-218): normal (0,0,1) seen head-on
+    gives intensity 1: colour = (0.01 + 1 + 1) * |n| = (0, 0, 2.01)."""
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    f = np.array([[0, 1, 2]], np.uint32)
+    face, rgba = O.ray_color(v, f, (0.2, 0.2, 1.0), (0, 0, -1), 10.0)
+    np.testing.assert_allclose(rgba[:3], [0, 0, 2.01], rtol=1e-6)
+    # 60 degrees off the normal, seen from behind: intensity 0.5, specular 0.5^64
+    d = np.array([0.0, math.sin(math.pi / 3), math.cos(math.pi / 3)], np.float32)
+    face, rgba = O.ray_color(v, f, np.array([0.2, 0.2, 0.0]) - d, d, 10.0)
+    assert face == 0
+    np.testing.assert_allclose(rgba[2], 0.01 + 0.5 + 0.5 ** 64, rtol=1e-5)
+
+
+# ---------------------------------------------------------------------------------------
+# Camera (src/main_view.wgsl:347-366)
+# ---------------------------------------------------------------------------------------
+def test_camera_centre_and_corner_rays():
+    cam = O.camera_in((0, 0, 3), (0, 0, -1), (0, 1, 0))
+    W, H = 101, 51
+    np.testing.assert_allclose(O.camera_ray(cam, W, H, 50, 25), [0, 0, -1], atol=1e-15)
+    # corner (0, 0): u = -vw/2, v = -vh/2 along horizontal = cross(up, w) = +x, vertical = +y
+    vh = 0.025
+    vw = W / H * vh
+    expect = np.array([-vw / 2, -vh / 2, -0.035])
+    np.testing.assert_allclose(O.camera_ray(cam, W, H, 0, 0), expect / np.linalg.norm(expect), rtol=1e-12)
+    # field-of-view extension: the top-centre ray is fov/2 above the axis
+    cam60 = O.camera_in((0, 0, 3), (0, 0, -1), (0, 1, 0), 60.0)
+    d = O.camera_ray(cam60, W, H, 50, H - 1)
+    assert math.degrees(math.atan2(d[1], -d[2])) == pytest.approx(30.0, abs=1e-9)
+
+
+# ---------------------------------------------------------------------------------------
+# refraction helpers (src/main_view.wgsl:111-136)
+# ---------------------------------------------------------------------------------------
+@pytest.mark.parametrize("T,n", [(15, 0.9993990), (20, 0.9823091), (30, 0.9498201), (100, 0.7711449),
+                                 (200, 0.6075901)])
+def test_refraction_index_values(T, n):
+    assert O.refraction_index(T) == pytest.approx(n, abs=5e-7)
+
+
+def test_refraction_snell_and_total_reflection():
+    n = np.array([0.0, 1.0, 0.0])
+    for theta in (0.0, 0.3, 0.7, 1.2):
+        v = np.array([math.sin(theta), -math.cos(theta), 0.0])  # travelling against n
+        r = O.refraction(200.0, 15.0, v, n)                      # hot -> cold: eta = n200/n15 < 1
+        eta = O.refraction_index(200.0) / O.refraction_index(15.0)
+        assert np.linalg.norm(r) == pytest.approx(1.0, abs=1e-12)
+        assert r[0] == pytest.approx(eta * math.sin(theta), abs=1e-12)   # Snell: tangential part * eta
+        assert r[1] < 0
+    # cold -> hot at grazing incidence: eta > 1 -> total internal reflection = mirror image
+    theta = 1.4
+    v = np.array([math.sin(theta), -math.cos(theta), 0.0])
+    r = O.refraction(15.0, 200.0, v, n)
+    np.testing.assert_allclose(r, [v[0], -v[1], 0.0], atol=1e-12)
+
+
+# ---------------------------------------------------------------------------------------
+# field bodies (src/functions.rs:33-128 and the benchmark fields)
+# ---------------------------------------------------------------------------------------
+def test_lorenz_and_roessler_fixed_points():
+    beta, rho = 8.0 / 3.0, 28.0
+    c = math.sqrt(beta * (rho - 1))
+    z = np.zeros(3)
+    for p in ([c, c, rho - 1], [-c, -c, rho - 1], [0, 0, 0]):
+        np.testing.assert_allclose(O.field(FIELDS["lorenz"], z, p, z, z, 0.0), 0, atol=1e-12)
+    np.testing.assert_allclose(O.field(FIELDS["lorenz"], z, [1, 2, 3], z, z, 0.0), [10, 23, 2 - 8], atol=1e-12)
+    a, b, cc = 0.1, 0.1, 14.0
+    disc = math.sqrt(cc * cc - 4 * a * b)
+    for s in (+1, -1):
+        x = (cc + s * disc) / 2
+        p = [x, -x / a, x / a]
+        np.testing.assert_allclose(O.field(FIELDS["roessler"], z, p, z, z, 0.0), 0, atol=1e-9)
+
+
+def test_translation_rotation_fields():
+    z = np.zeros(3)
+    v0 = np.array([0.3, -0.2, 0.9])
+    np.testing.assert_allclose(O.field(FIELDS["translation_x"], z, z, v0, z, 2.0), v0 + [1.0, 0, 0])
+    np.testing.assert_allclose(O.field(FIELDS["translation_z"], z, z, v0, z, 2.0), v0 + [0, 0, 1.0])
+    # rotation: rotateZ drops z (zero third row), so at t = 0: normalize(v0 + (v0.x, v0.y, 0))
+    r0 = O.field(FIELDS["rotation"], z, z, v0, z, 0.0)
+    e = v0 + [v0[0], v0[1], 0.0]
+    np.testing.assert_allclose(r0, e / np.linalg.norm(e), atol=1e-14)
+    # at t = 1 (phi = pi): rotateZ(v0) = (-x, -y, 0) -> normalize((0, 0, z))
+    np.testing.assert_allclose(O.field(FIELDS["rotation"], z, z, v0, z, 1.0), [0, 0, 1], atol=1e-12)
+
+
+def test_swirl_lens_warp_fields():
+    z = np.zeros(3)
+    v0 = np.array([0.0, 0.1, -1.0])
+    p = np.array([0.0, 0.0, 2.0])          # |p| = 2 -> rotation by 1 rad about +y
+    f = O.field(FIELDS["swirl"], z, p, v0, z, 0.0)
+    np.testing.assert_allclose(f, [math.sin(1.0) * -1.0, 0.1, math.cos(1.0) * -1.0], atol=1e-14)
+    v = np.array([0.0, 0.0, -1.0])
+    np.testing.assert_allclose(O.field(FIELDS["lens"], z, [2.0, 0, 0], z, v, 0.0), v - [0.3 * 2 / 8, 0, 0])
+    np.testing.assert_allclose(O.field(FIELDS["lens"], z, [0.01, 0, 0], z, v, 0.0), v - [0.3 * 0.01 / 1e-3, 0, 0])
+    q = np.array([math.pi / 8, math.pi / 8, 0.0])   # sin(4 * pi/8) = 1
+    np.testing.assert_allclose(O.field(FIELDS["warp"], z, q, v0, z, 0.0), v0 + [0.3, 0, 0.3], atol=1e-14)
+
+
+def test_mirage_and_hot_air_temperature_regimes():
+    v = np.array([0.3, -0.5, 0.8]) / np.linalg.norm([0.3, -0.5, 0.8])
+    far = np.array([3.0, 3.0, 3.0])
+    for fid in (1, 3, 19):   # far from the source: equal temperatures -> eta = 1 -> v
+        np.testing.assert_allclose(O.field(fid, far, far, v, v, 0.0), v, atol=1e-12)
+    # plane mirages: beyond max_dist from the plane y = 0.1 the medium is uniform
+    for fid in (2, 4):
+        p = np.array([0.0, 0.5, 0.0])
+        np.testing.assert_allclose(O.field(fid, p, p, v, v, 0.0), v, atol=1e-12)
+    # crossing from hot (at the plane) to cold: the ray bends away from the normal
+    pin, pout = np.array([0, 0.1, 0]), np.array([0, 0.12, 0])
+    r = O.field(2, pin, pout, v, v, 0.0)
+    assert abs(r[0]) < abs(v[0]) or abs(r[1]) != abs(v[1])
+
+
+# ---------------------------------------------------------------------------------------
+# RK4 loop (src/main_view.wgsl:229-279)
+# ---------------------------------------------------------------------------------------
+def _ig(**k):
+    base = dict(h_initial=0.01, t_max=5.0, adaptive=False, max_steps=100000, samples_per_ray=0, sample_steps=0)
+    base.update(k)
+    return O.integrator_in(**base)
+
+
+def test_f32_step_count_is_the_f32_running_sum():
+    for h in (0.1, 0.01, 0.005, 0.003):
+        t = np.float32(0)
+        n = 0
+        while t <= np.float32(5.0):
+            t = np.float32(t + np.float32(h))
+            n += 1
+        steps, _, _ = O.trace(FIELDS["identity"], _ig(h_initial=h), (0, 0, 0), (0, 0, -1), precision="f32")
+        assert steps == n, (h, steps, n)
+
+
+def test_identity_field_is_a_straight_line():
+    d = np.array([0.2, -0.1, -1.0])
+    d /= np.linalg.norm(d)
+    steps, end, face = O.trace(FIELDS["identity"], _ig(h_initial=0.01), (0.1, 0.2, 3.0), d)
+    t_final = steps * 0.01
+    np.testing.assert_allclose(end, np.array([0.1, 0.2, 3.0]) + d * t_final, atol=1e-12)
+
+
+def test_identity_field_hits_like_a_linear_ray_cast():
+    v, f, c = scenes.reference_startup_scene()
+    cam = O.as_oracle_struct(Camera.arcball_default(c), O.CameraIn)
+    W, H = 16, 12
+    r = O.render(cam, O.settings_in(), _ig(h_initial=0.01), FIELDS["identity"], v, f, W, H, threads=4)
+    for gy in range(H):
+        for gx in range(W):
+            d = O.camera_ray(cam, W, H, gx, gy).astype(np.float32)
+            face, _ = O.ray_color(v, f, np.array(cam.origin[:3], np.float32), d, 100.0)
+            assert r["hit"][H - 1 - gy, gx] == face
+
+
+def test_translation_field_rk4_is_exact():
+    """f = v0 + (0.5 t, 0, 0) is polynomial in t, so RK4 reproduces p(t) = p0 + v0 t + (t^2/4, 0, 0)."""
+    d = np.array([0.0, 0.6, -0.8])
+    steps, end, _ = O.trace(FIELDS["translation_x"], _ig(h_initial=0.0625, t_max=1e9, max_steps=32), (0, 0, 0), d)
+    t = 32 * 0.0625
+    np.testing.assert_allclose(end, d * t + [t * t / 4, 0, 0], atol=1e-12)
+
+
+@pytest.mark.parametrize("field", ["swirl", "warp"])
+def test_rk4_fourth_order_convergence(field):
+    """Fields that depend on p and v0 only give p' = f(p): halving h must cut the error ~16x."""
+    d = np.array([0.1, 0.05, -1.0])
+    d /= np.linalg.norm(d)
+    o = np.array([0.2, 0.1, 1.5])
+    T = 2.0
+    ends = {}
+    for k in (3, 4, 5, 8):
+        h = 2.0 ** -k
+        _, ends[k], _ = O.trace(FIELDS[field], _ig(h_initial=h, t_max=1e9, max_steps=int(T / h)), o, d)
+    e3 = np.linalg.norm(ends[3] - ends[8])
+    e4 = np.linalg.norm(ends[4] - ends[8])
+    e5 = np.linalg.norm(ends[5] - ends[8])
+    assert 12 < e3 / e4 < 20 and 12 < e4 / e5 < 20, (e3, e4, e5)
+
+
+def test_adaptive_refinement_triggers_on_a_jump():
+    """A ray crossing the plane mirage's 0.01-thick hot layer (y = 0.1) sees |V - last_V| jump
+    more than 10x: with adaptive sampling the loop redoes that step at h = 0.001."""
+    o, d = (0.0, 0.5, 0.0), (0.0, -1.0, 0.05)
+    n_fixed, end_fixed, _ = O.trace(FIELDS["mirage_plane"], _ig(h_initial=0.1, adaptive=False), o, d)
+    n_adapt, end_adapt, _ = O.trace(FIELDS["mirage_plane"], _ig(h_initial=0.1, adaptive=True), o, d)
+    assert n_fixed == 51
+    assert n_adapt > n_fixed
+    assert not np.allclose(end_fixed, end_adapt)
+
+
+def test_escape_mode_stops_outside_the_sphere():
+    steps, end, _ = O.trace(FIELDS["identity"], _ig(h_initial=0.01, mode=1, escape_radius=2.0), (0, 0, 3.0),
+                            (0, 1.0, 0.0))
+    assert steps == 1 and end[1] == pytest.approx(0.01)
+    steps, end, _ = O.trace(FIELDS["identity"], _ig(h_initial=0.01, mode=1, escape_radius=2.0), (0, 0, 3.0),
+                            (0, 0.0, -1.0))
+    # enters at z = 2, leaves at z = -2 moving outward: first step with |p| > 2 and p.v > 0
+    assert end[2] < -2.0 and end[2] > -2.0 - 0.0101
+
+
+# ---------------------------------------------------------------------------------------
+# Lyapunov overlay (src/overlay.wgsl:4-146)
+# ---------------------------------------------------------------------------------------
+def test_lyapunov_equals_largest_singular_value():
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        A = rng.normal(size=(3, 2)) * rng.uniform(0.01, 3)
+        b = rng.normal(size=3)
+        m = lambda x, y: A @ np.array([x, y]) + b
+        for delta in (1, 3):
+            lam = O.lyapunov(m(delta, 0), m(-delta, 0), m(0, delta), m(0, -delta))
+            s = np.linalg.svd(A, compute_uv=False)
+            assert lam == pytest.approx(delta * s[0], rel=1e-6, abs=1e-9)
+
+
+def test_overlay_blend_and_exponent_layout():
+    W, H, delta = 12, 9, 1
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float64)
+    mapping = np.zeros((H, W, 4), np.float32)
+    lam = 0.098
+    mapping[..., 0] = lam * xx      # lambda = 0.098 everywhere inside
+    mapping[..., 3] = 1
+    color = np.full((H, W, 4), 100, np.uint8)
+    color[..., 3] = 255
+    st = O.settings_in(overlay_mode=1, delta=delta, scaling=50.0)
+    ov = O.overlay(st, color, mapping)
+    s = math.exp(50 * lam - 5)        # 0.905 >= 0.8: blend with alpha = s / 2
+    inside = ov["lyapunov"] >= 0
+    assert inside.sum() == (W - 4) * (H - 4)
+    np.testing.assert_allclose(ov["lyapunov"][inside], lam, rtol=1e-6)
+    alpha = 0.5 * s
+    exp_rgb = round((alpha + (1 - alpha) * 100 / 255) * 255)
+    assert np.all(ov["final"][inside][:, :3] == exp_rgb)
+    assert np.all(ov["final"][inside][:, 3] == 255)
+    assert np.all(ov["final"][~inside] == color[~inside])
+    # row stride of the exponents buffer is the height (src/overlay.wgsl:128)
+    y, x = 3, 5
+    assert ov["exponents"][H * y + x] == pytest.approx(s, rel=1e-5)
+    # below the threshold: colour copied
+    mapping[..., 0] = 0.05 * xx
+    ov = O.overlay(st, color, mapping)
+    assert np.array_equal(ov["final"], color)
+    # overlay disabled: copy
+    ov = O.overlay(O.settings_in(overlay_mode=0), color, mapping)
+    assert np.array_equal(ov["final"], color)
+
+
+def test_sample_rays_identity_are_straight():
+    cam = O.camera_in((0, 0, 2), (0, 0, -1), (0, 1, 0))
+    st = O.settings_in(mouse_pos=(0.5, 0.5))
+    ig = O.integrator_in(samples_per_ray=100, sample_steps=5000)
+    out = O.sample_rays(cam, st, ig, FIELDS["identity"], 64, 48, precision="f64")
+    W, H = 64, 48
+    vh, vw = 0.025, 64 / 48 * 0.025
+    offs = [(-1, -1), (0, -1), (1, -1), (1, 0), (1, 1), (0, 1), (-1, 1), (-1, 0)]
+    for r, (ox, oy) in enumerate(offs):
+        px, py = 0.5 + 0.01 * ox, 0.5 + 0.01 * oy
+        d = np.array([px * vw - vw / 2, py * vh - vh / 2, -0.035])
+        d /= np.linalg.norm(d)
+        for k in (0, 1, 57, 99):
+            np.testing.assert_allclose(out[r, k, 0, :3], np.array([0, 0, 2.0]) + d * (k * 50 * 0.001), atol=1e-9)
+        assert out[r, 0, 1, 3] == 0.5
+    np.testing.assert_allclose(out[0, 0, 1, :3], [1, 0, 0])
+    np.testing.assert_allclose(out[7, 0, 1, :3], [0.5, 0, 0.5])
+
+
+def test_prefilter_does_not_change_results():
+    v, f = scenes.benchmark_scene(level=1)
+    cam = O.camera_in((0.3, 0.2, 3.0), (-0.1, -0.06, -1.0), (0, 1, 0), 60.0)
+    res = []
+    for on in (False, True):
+        O.set_prefilter(on)
+        res.append(O.render(cam, O.settings_in(), _ig(h_initial=0.01), FIELDS["warp"], v, f, 24, 16, threads=4))
+    O.set_prefilter(True)
+    for k in ("hit", "mapping", "color8", "steps"):
+        assert np.array_equal(res[0][k], res[1][k]), k
+    assert res[1]["tests"] < res[0]["tests"]
