@@ -164,6 +164,12 @@ linon_status linon_set_field_predefined(linon_context* ctx, int field);
  * :852-861), translated to HIP and compiled with hiprtc.  LINON_ERR_COMPILE + message on a
  * bad body; the previous field stays active. */
 linon_status linon_set_field_wgsl(linon_context* ctx, const char* body);
+/* The two halves of linon_set_field_wgsl without a device (no counterpart in the reference,
+ * where naga validates at create_shader_module, src/main_view.rs:662-665): the HIP C++ a body
+ * translates to, and a full hiprtc compile of the ray-cast kernel with it.  On failure the
+ * message (translator error or hiprtc log) is written to out / log. */
+linon_status linon_wgsl_translate(const char* body, char* out, size_t capacity, size_t* needed);
+linon_status linon_wgsl_compile_check(const char* body, char* log, size_t capacity);
 linon_status linon_set_integrator(linon_context* ctx, const linon_integrator* integrator);
 linon_status linon_get_integrator(const linon_context* ctx, linon_integrator* integrator);
 

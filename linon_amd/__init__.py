@@ -169,6 +169,8 @@ SIGNATURES = {
     "linon_set_accel": (C.c_int, [_P, C.c_int, C.c_int]),
     "linon_set_field_predefined": (C.c_int, [_P, C.c_int]),
     "linon_set_field_wgsl": (C.c_int, [_P, C.c_char_p]),
+    "linon_wgsl_translate": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "linon_wgsl_compile_check": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
     "linon_set_integrator": (C.c_int, [_P, C.POINTER(Integrator)]),
     "linon_get_integrator": (C.c_int, [_P, C.POINTER(Integrator)]),
     "linon_render": (C.c_int, [_P, _P]),
@@ -207,6 +209,32 @@ def load_library() -> C.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def wgsl_translate(body: str) -> str:
+    """HIP C++ for a WGSL field body (no GPU needed); raises LinonError with the translator's
+    line:col message for a body outside the supported WGSL subset."""
+    lib = load_library()
+    need = C.c_size_t(0)
+    buf = C.create_string_buffer(1 << 16)
+    st = lib.linon_wgsl_translate(body.encode(), buf, len(buf), C.byref(need))
+    if need.value > len(buf):
+        buf = C.create_string_buffer(need.value)
+        st = lib.linon_wgsl_translate(body.encode(), buf, len(buf), C.byref(need))
+    if st != LINON_OK:
+        raise LinonError(st, buf.value.decode())
+    return buf.value.decode()
+
+
+def wgsl_compile_check(body: str) -> str:
+    """Translates and compiles a field body into the gfx950 ray-cast kernel with hiprtc (no GPU
+    needed).  Returns a short summary, raises LinonError with the log on failure."""
+    lib = load_library()
+    buf = C.create_string_buffer(1 << 20)
+    st = lib.linon_wgsl_compile_check(body.encode(), buf, len(buf))
+    if st != LINON_OK:
+        raise LinonError(st, buf.value.decode(errors="replace"))
+    return buf.value.decode()
 
 
 def _ptr(a: np.ndarray) -> C.c_void_p:

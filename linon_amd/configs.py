@@ -67,3 +67,18 @@ def setup_view(view, wl: Workload, field) -> None:
     view.update_settings(wl.settings())
     view.set_integrator(wl.integrator())
     view.set_field(field)
+
+
+# The four benchmark fields as WGSL field bodies (what a linon user would type into the
+# field editor); linon_set_field_wgsl compiles them to the same arithmetic as the predefined
+# functors LINON_FIELD_SWIRL / LENS / HOT_AIR / WARP.
+FIELD_WGSL = {
+    "swirl": "return rotateY(v0, 0.5 * length(p));",
+    "lens": "let r = length(p);\nreturn v - 0.3 * p / max(r * r * r, 0.001);",
+    "hot_air": (
+        "let c = vec3<f32>(0.0, -0.5, 0.0);\n"
+        "let temp_prev = 20.0 + 80.0 * sigmoid(8.0 * (0.5 - length(p_prev - c)));\n"
+        "let temp = 20.0 + 80.0 * sigmoid(8.0 * (0.5 - length(p - c)));\n"
+        "return refraction(temp_prev, temp, v, normalize(p - c));"),
+    "warp": "return v0 + 0.3 * vec3<f32>(sin(4.0 * p.y), 0.0, sin(4.0 * p.x));",
+}

@@ -92,6 +92,22 @@ __device__ __forceinline__ float smoothstep(float e0, float e1, float x) {
     return t * t * (3.0f - 2.0f * t);
 }
 __device__ __forceinline__ float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ int abs(int x) { return x < 0 ? -x : x; }
+__device__ __forceinline__ int sign(int x) { return (x > 0) - (x < 0); }
+__device__ __forceinline__ int clamp(int e, int lo, int hi) { return min(max(e, lo), hi); }
+__device__ __forceinline__ unsigned min(unsigned a, unsigned b) { return a < b ? a : b; }
+__device__ __forceinline__ unsigned max(unsigned a, unsigned b) { return a > b ? a : b; }
+__device__ __forceinline__ unsigned clamp(unsigned e, unsigned lo, unsigned hi) { return min(max(e, lo), hi); }
+// WGSL % : truncated remainder for floats (a - b * trunc(a / b)), C remainder for integers
+__device__ __forceinline__ float wmod(float a, float b) { return a - b * __builtin_truncf(a / b); }
+__device__ __forceinline__ vec3 wmod(vec3 a, vec3 b) { return {wmod(a.x, b.x), wmod(a.y, b.y), wmod(a.z, b.z)}; }
+__device__ __forceinline__ int wmod(int a, int b) { return b == 0 ? a : a % b; }
+__device__ __forceinline__ unsigned wmod(unsigned a, unsigned b) { return b == 0u ? a : a % b; }
+template <int X, int Y, int Z>
+__device__ __forceinline__ vec3 swizzle3(vec3 v) {
+    const float c[3] = {v.x, v.y, v.z};
+    return {c[X], c[Y], c[Z]};
+}
 
 // ---- helpers of src/main_view.wgsl:76-146 ----------------------------------------------
 constexpr float PI = 3.141592653589793f;

@@ -218,3 +218,31 @@ def test_config2_full_resolution_sample(field):
     assert info["gated"] > 100, info
     assert info["hit_ok"], info
     assert info["exit_ok"] >= HIT_MIN_AGREE, info
+
+
+@pytest.mark.parametrize("field", ["swirl", "lens", "hot_air", "warp"])
+def test_wgsl_body_matches_predefined_functor(view, field):
+    """MainView::reload_shader path: the WGSL body of a benchmark field, translated and compiled
+    with hiprtc, renders the same frame as the statically compiled functor."""
+    wl = configs.CONFIG2
+    v, f = scenes.benchmark_scene(level=2)
+    view.resize_texture(64, 40)
+    print(f"[case] wgsl-vs-functor {field}", flush=True)
+    a = run_gpu(view, wl.camera(), wl.settings(), wl.integrator(), field, v, f)
+    b = run_gpu(view, wl.camera(), wl.settings(), wl.integrator(), configs.FIELD_WGSL[field], v, f)
+    assert np.array_equal(a.hit_faces, b.hit_faces)
+    assert np.array_equal(a.steps, b.steps)
+    d = np.abs(a.mapping.astype(np.float64) - b.mapping) / np.maximum(1.0, np.abs(a.mapping))
+    assert d.max() <= 1e-5, d.max()
+
+
+def test_wgsl_compile_error_keeps_previous_field(view):
+    v, f, c = scenes.reference_startup_scene()
+    view.resize_texture(32, 24)
+    a = run_gpu(view, L.Camera.arcball_default(c), L.Settings.make(), L.Integrator.make(), "identity", v, f)
+    with pytest.raises(L.LinonError) as e:
+        view.reload_shader("return v +;")
+    assert e.value.status == 2 and "field_function:1" in e.value.message
+    view.render()
+    b = view.read_frame()
+    assert np.array_equal(a.mapping, b.mapping)
