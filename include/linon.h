@@ -137,6 +137,26 @@ typedef struct linon_device_buffers {
 
 typedef struct linon_context linon_context;
 
+/* One view of a batch: a camera, a resolution, a predefined field and a step size.  The
+ * reference renders one view per MainView; a batch renders many views (and step sizes, e.g. an
+ * "Enhance" pass at h/4) in one persistent launch per field, captured into a hipGraph. */
+typedef struct linon_view {
+    linon_camera camera;
+    uint32_t width, height;
+    int32_t field;        /* predefined field id (linon_field) */
+    float h_initial;
+    float field_weight;
+    int32_t pad[3];
+} linon_view;
+
+/* Launch shape of the persistent batch kernel, chosen by linon_autotune_batch. */
+typedef struct linon_batch_tuning {
+    int32_t waves_per_cu;     /* persistent grid = waves_per_cu x compute units */
+    int32_t waves_per_block;  /* 1, 2 or 4 wave64 per workgroup */
+    int32_t compact_every;    /* RK4 iterations a wave runs between refills */
+    int32_t refill_min;       /* idle lanes (of 64) that trigger a refill */
+} linon_batch_tuning;
+
 /* MainView::new (src/main_view.rs:82-458): allocates the outputs for a width x height view,
  * sets the reference's initial settings (:133-139), integrator (h 0.1, t 5, adaptive) and
  * field (MirageSphericalSigmoid, :100).  device = HIP device ordinal. */
@@ -178,6 +198,25 @@ linon_status linon_render(linon_context* ctx, void* stream);
 /* MainView::render_high_accuracy (src/main_view.rs:778-802): one render at h_initial = 0.001. */
 linon_status linon_render_high_accuracy(linon_context* ctx, void* stream);
 linon_status linon_synchronize(linon_context* ctx);
+
+/* Batches (no counterpart in the reference, which renders one view per MainView::render).
+ * linon_set_batch uploads the views, allocates their outputs and captures the batch (queue
+ * resets + one persistent launch per field) into a hipGraph; linon_render_batch replays it.
+ * The integration loop (mode, t_max, max_steps, adaptive, escape radius) is the context's
+ * integrator; the scene is the context's mesh.  Sample rays and the overlay are not part of
+ * a batch. */
+linon_status linon_set_batch(linon_context* ctx, const linon_view* views, int num_views);
+linon_status linon_render_batch(linon_context* ctx, void* stream);
+linon_status linon_read_batch_view(linon_context* ctx, int view, uint8_t* rgba, size_t rgba_bytes, float* mapping,
+                                   size_t mapping_bytes, int32_t* hit_faces, size_t hit_bytes, int32_t* steps,
+                                   size_t steps_bytes);
+linon_status linon_set_batch_tuning(linon_context* ctx, const linon_batch_tuning* tuning);
+linon_status linon_get_batch_tuning(const linon_context* ctx, linon_batch_tuning* tuning);
+/* Times candidate launch shapes on a copy of the current batch with every view scaled to
+ * 1/`downscale` of its width and height, keeps the fastest (coordinate descent over waves per
+ * CU, waves per block, steps per compaction round and refill threshold) and re-captures the
+ * batch with it.  best_ms: the proxy batch's time with the chosen shape. */
+linon_status linon_autotune_batch(linon_context* ctx, int downscale, linon_batch_tuning* best, float* best_ms);
 
 /* Read-back (the reference keeps these on the GPU; sizes are checked against W*H). */
 linon_status linon_read_final(linon_context* ctx, uint8_t* rgba, size_t bytes);

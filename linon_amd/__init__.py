@@ -151,7 +151,31 @@ class DeviceBuffers(C.Structure):
                 ("height", C.c_uint32)]
 
 
+class View(C.Structure):
+    """One view of a batch (linon_view)."""
+    _fields_ = [("camera", Camera), ("width", C.c_uint32), ("height", C.c_uint32), ("field", C.c_int32),
+                ("h_initial", C.c_float), ("field_weight", C.c_float), ("pad", C.c_int32 * 3)]
+
+    @classmethod
+    def make(cls, camera: Camera, width: int, height: int, field, h_initial: float, field_weight: float = 1.0):
+        v = cls()
+        v.camera = camera
+        v.width, v.height = width, height
+        v.field = FIELDS[field] if isinstance(field, str) else int(field)
+        v.h_initial, v.field_weight = h_initial, field_weight
+        return v
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("waves_per_cu", C.c_int32), ("waves_per_block", C.c_int32), ("compact_every", C.c_int32),
+                ("refill_min", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 assert C.sizeof(Camera) == 112 and C.sizeof(Settings) == 24 and C.sizeof(Integrator) == 48
+assert C.sizeof(View) == 144 and C.sizeof(BatchTuning) == 16
 
 _lib = None
 
@@ -187,6 +211,13 @@ SIGNATURES = {
     "linon_get_counters": (C.c_int, [_P, C.POINTER(Counters)]),
     "linon_get_device_buffers": (C.c_int, [_P, C.POINTER(DeviceBuffers)]),
     "linon_get_timings": (C.c_int, [_P, _P, _P, C.c_int, C.POINTER(C.c_int)]),
+    "linon_set_batch": (C.c_int, [_P, _P, C.c_int]),
+    "linon_render_batch": (C.c_int, [_P, _P]),
+    "linon_read_batch_view": (C.c_int, [_P, C.c_int, _P, C.c_size_t, _P, C.c_size_t, _P, C.c_size_t, _P,
+                                        C.c_size_t]),
+    "linon_set_batch_tuning": (C.c_int, [_P, C.POINTER(BatchTuning)]),
+    "linon_get_batch_tuning": (C.c_int, [_P, C.POINTER(BatchTuning)]),
+    "linon_autotune_batch": (C.c_int, [_P, C.c_int, C.POINTER(BatchTuning), C.POINTER(C.c_float)]),
     "linon_reset_timings": (C.c_int, [_P]),
 }
 
@@ -368,6 +399,42 @@ class MainView:
         c = Counters()
         self._check(self._lib.linon_get_counters(self._h, C.byref(c)))
         return {k: getattr(c, k) for k, _ in Counters._fields_}
+
+    # -- batches of views (one persistent launch per field, captured in a hipGraph) ----------
+    def set_batch(self, views) -> None:
+        arr = (View * len(views))(*views)
+        self._batch = list(views)
+        self._check(self._lib.linon_set_batch(self._h, C.cast(arr, C.c_void_p), len(views)))
+
+    def render_batch(self, stream: int | None = None) -> None:
+        self._check(self._lib.linon_render_batch(self._h, C.c_void_p(stream)))
+
+    def read_batch_view(self, i: int) -> Frame:
+        v = self._batch[i]
+        H, W = v.height, v.width
+        rgba = np.empty((H, W, 4), np.uint8)
+        mapping = np.empty((H, W, 4), np.float32)
+        hit = np.empty((H, W), np.int32)
+        steps = np.empty((H, W), np.int32)
+        self._check(self._lib.linon_read_batch_view(self._h, i, _ptr(rgba), rgba.nbytes, _ptr(mapping), mapping.nbytes,
+                                                    _ptr(hit), hit.nbytes, _ptr(steps), steps.nbytes))
+        empty = np.zeros(0, np.float32)
+        return Frame(rgba, rgba, mapping, hit, steps, empty, empty, {})
+
+    def batch_tuning(self) -> BatchTuning:
+        t = BatchTuning()
+        self._check(self._lib.linon_get_batch_tuning(self._h, C.byref(t)))
+        return t
+
+    def set_batch_tuning(self, waves_per_cu=16, waves_per_block=1, compact_every=32, refill_min=16) -> None:
+        t = BatchTuning(waves_per_cu, waves_per_block, compact_every, refill_min)
+        self._check(self._lib.linon_set_batch_tuning(self._h, C.byref(t)))
+
+    def autotune_batch(self, downscale: int = 4):
+        t = BatchTuning()
+        ms = C.c_float(0)
+        self._check(self._lib.linon_autotune_batch(self._h, downscale, C.byref(t), C.byref(ms)))
+        return t, ms.value
 
     def timings(self):
         """(raycast_ms, render_ms) arrays of the renders since reset_timings()."""

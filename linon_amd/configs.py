@@ -82,3 +82,43 @@ FIELD_WGSL = {
         "return refraction(temp_prev, temp, v, normalize(p - c));"),
     "warp": "return v0 + 0.3 * vec3<f32>(sin(4.0 * p.y), 0.0, sin(4.0 * p.x));",
 }
+
+
+# ---------------------------------------------------------------------------------------------
+# 64 views in one persistent launch per field (escape mode, uneven path lengths)
+# ---------------------------------------------------------------------------------------------
+BATCH_RESOLUTIONS = ((256, 256), (512, 512), (1024, 1024), (1920, 1080), (2048, 2048))
+BATCH_STEPS = (0.01, 0.005, 0.0025)
+
+
+def batch_views(num_views: int = 64, seed: int = 0, fov_deg: float = 60.0, enhance: bool = True,
+                scale: int = 1):
+    """Views drawn from numpy's PCG64(seed): resolution uniform over BATCH_RESOLUTIONS (divided
+    by `scale`), eye uniform on a sphere of radius U(2.5, 4) looking at the origin, field
+    uniform over the four benchmark fields, h uniform over BATCH_STEPS; plus one 'Enhance' view
+    (a copy of view 0 at h/4) when `enhance`."""
+    import numpy as np
+    from . import View
+
+    rng = np.random.default_rng(seed)
+    views = []
+    for _ in range(num_views):
+        w, h = BATCH_RESOLUTIONS[int(rng.integers(len(BATCH_RESOLUTIONS)))]
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        eye = d * rng.uniform(2.5, 4.0)
+        field = ("swirl", "lens", "hot_air", "warp")[int(rng.integers(4))]
+        step = BATCH_STEPS[int(rng.integers(len(BATCH_STEPS)))]
+        cam = Camera.look_at(eye, (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), fov_deg)
+        views.append(View.make(cam, max(8, w // scale), max(8, h // scale), field, step))
+    if enhance:
+        v0 = views[0]
+        views.append(View.make(v0.camera, v0.width, v0.height, v0.field, v0.h_initial / 4.0))
+    return views
+
+
+def batch_integrator(max_steps: int = 8000) -> Integrator:
+    """Escape mode: a ray stops when it leaves the radius-2 sphere moving outward, at its first
+    hit (no overlay in a batch), or after max_steps RK4 iterations."""
+    return Integrator.make(h_initial=0.01, t_max=1e30, adaptive=False, max_steps=max_steps, mode=1,
+                           escape_radius=2.0, samples_per_ray=0, sample_steps=0, need_mapping=False)

@@ -82,50 +82,59 @@ linon_status Context::alloc_samples() {
     return LINON_OK;
 }
 
-float Context::viewport_height() const {
+void Context::fill_scene(SceneParams& S) const {
+    std::memset(&S, 0, sizeof(S));
+    S.tris = d_tris;
+    S.num_faces = (int)num_faces;
+    S.accel = (accel_req == LINON_ACCEL_BRUTE) ? LINON_ACCEL_BRUTE_K : LINON_ACCEL_GRID_K;
+    S.cell_start = d_cell_start;
+    S.cell_faces = d_cell_faces;
+    S.dist = d_dist;
+    for (int k = 0; k < 3; ++k) {
+        S.grid_lo[k] = grid_lo[k];
+        S.grid_hi[k] = grid_hi[k];
+        S.gdim[k] = gdim[k];
+    }
+    S.cell_size = cell_size;
+    S.inv_cell_size = 1.0f / cell_size;
+    S.skip_margin = skip_margin;
+}
+
+void fill_step(StepParams& I, const linon_integrator& ig, float field_weight) {
+    I.h_initial = ig.h_initial;
+    I.t_max = ig.t_max;
+    I.field_weight = field_weight;
+    I.escape_r2 = ig.escape_radius * ig.escape_radius;
+    I.adaptive = ig.adaptive;
+    I.max_steps = ig.max_steps;
+    I.mode = ig.mode;
+    I.need_mapping = ig.need_mapping;
+}
+
+void fill_camera(CameraParams& C, const linon_camera& cam, uint32_t w, uint32_t h) {
+    C.width = (int)w;
+    C.height = (int)h;
+    for (int k = 0; k < 3; ++k) {
+        C.origin[k] = cam.origin[k];
+        C.view_dir[k] = cam.view_direction[k];
+        C.up[k] = cam.up[k];
+    }
     // reference: fixed 0.025 at focal length 0.035 (src/main_view.wgsl:355-356)
-    if (camera.view_direction[3] > 0.0f)
-        return 2.0f * 0.035f * (float)std::tan((double)camera.view_direction[3] * 3.141592653589793 / 360.0);
-    return 0.025f;
+    C.viewport_h = cam.view_direction[3] > 0.0f
+                       ? 2.0f * 0.035f * (float)std::tan((double)cam.view_direction[3] * 3.141592653589793 / 360.0)
+                       : 0.025f;
 }
 
 void Context::fill_raycast_params(RaycastParams& P, const linon_integrator& ig) const {
     std::memset(&P, 0, sizeof(P));
-    P.width = (int)width;
-    P.height = (int)height;
+    fill_camera(P.cam, camera, width, height);
+    fill_step(P.step, ig, settings.field_weight);
+    fill_scene(P.scene);
     P.tiles_x = (int)((width + LINON_TILE - 1) / LINON_TILE);
     P.tiles_y = (int)((height + LINON_TILE - 1) / LINON_TILE);
     P.sample_block = (ig.samples_per_ray > 0 && ig.sample_steps > 0) ? 1 : 0;
-    for (int k = 0; k < 3; ++k) {
-        P.origin[k] = camera.origin[k];
-        P.view_dir[k] = camera.view_direction[k];
-        P.up[k] = camera.up[k];
-    }
-    P.viewport_h = viewport_height();
-    P.field_weight = settings.field_weight;
     P.mouse_x = settings.mouse_pos[0];
     P.mouse_y = settings.mouse_pos[1];
-    P.h_initial = ig.h_initial;
-    P.t_max = ig.t_max;
-    P.adaptive = ig.adaptive;
-    P.max_steps = ig.max_steps;
-    P.mode = ig.mode;
-    P.need_mapping = ig.need_mapping;
-    P.escape_r2 = ig.escape_radius * ig.escape_radius;
-    P.tris = d_tris;
-    P.num_faces = (int)num_faces;
-    P.accel = (accel_req == LINON_ACCEL_BRUTE) ? LINON_ACCEL_BRUTE_K : LINON_ACCEL_GRID_K;
-    P.cell_start = d_cell_start;
-    P.cell_faces = d_cell_faces;
-    P.dist = d_dist;
-    for (int k = 0; k < 3; ++k) {
-        P.grid_lo[k] = grid_lo[k];
-        P.grid_hi[k] = grid_hi[k];
-        P.gdim[k] = gdim[k];
-    }
-    P.cell_size = cell_size;
-    P.inv_cell_size = 1.0f / cell_size;
-    P.skip_margin = skip_margin;
     P.color = d_color;
     P.mapping = d_mapping;
     P.hit_face = d_hit;
@@ -156,8 +165,8 @@ linon_status Context::render_with(const linon_integrator& ig, hipStream_t s) {
     RaycastParams P;
     fill_raycast_params(P, ig);
     const unsigned blocks = (unsigned)(P.tiles_x * P.tiles_y + P.sample_block);
-    const size_t shmem = (P.accel == LINON_ACCEL_BRUTE_K && P.num_faces <= LINON_MAX_BRUTE_LDS)
-                             ? (size_t)P.num_faces * 48
+    const size_t shmem = (P.scene.accel == LINON_ACCEL_BRUTE_K && P.scene.num_faces <= LINON_MAX_BRUTE_LDS)
+                             ? (size_t)P.scene.num_faces * 48
                              : 0;
     LN_CHECK_HIP(this, launch_clear_u64(d_counters, LINON_NUM_COUNTERS, s));
     hipEvent_t* slot = next_timing_slot();
@@ -177,6 +186,7 @@ linon_status Context::render_with(const linon_integrator& ig, hipStream_t s) {
     O.overlay_mode = settings.overlay_mode;
     O.delta = settings.central_difference_delta;
     O.scaling = settings.lyapunov_scaling;
+    O.srgb = srgb_output ? 1 : 0;
     O.ray_casting = d_color;
     O.mapping = d_mapping;
     O.final_color = d_final;
@@ -274,6 +284,8 @@ linon_status linon_create(linon_context** out, int device, uint32_t width, uint3
     c->camera.view_direction[2] = -1.0f;
     c->camera.up[1] = 1.0f;
     for (int i = 0; i < 4; ++i) c->camera.view_matrix[5 * i] = 1.0f;
+    if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        c->num_cus = 256;
     if (hipMalloc(&c->d_counters, 8 * LINON_NUM_COUNTERS) != hipSuccess) {
         delete c;
         return LINON_ERR_HIP;
@@ -295,6 +307,7 @@ void linon_destroy(linon_context* ctx) {
     c->free_images();
     c->free_mesh();
     c->release_jit();
+    c->batch.release();
     if (c->d_samples) (void)hipFree(c->d_samples);
     if (c->d_counters) (void)hipFree(c->d_counters);
     for (auto& e : c->ring)

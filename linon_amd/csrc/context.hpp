@@ -10,6 +10,30 @@
 
 namespace linon {
 
+// One persistent launch of a batch: the views of one field share a work queue.
+struct BatchGroup {
+    int field = 0;
+    int first_view = 0;  // index into BatchState::d_views
+    int num_views = 0;
+    unsigned long long total_rays = 0;
+};
+
+struct BatchState {
+    std::vector<linon_view> views;  // caller's order
+    std::vector<size_t> view_px;    // pixel offset of each view in the arenas (+ total)
+    size_t total_px = 0;
+    std::vector<BatchGroup> groups;
+    BatchView* d_views = nullptr;   // grouped by field
+    unsigned long long* d_queues = nullptr;
+    unsigned* d_color = nullptr;
+    float* d_mapping = nullptr;
+    int* d_hit = nullptr;
+    int* d_steps = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void release();
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -67,11 +91,23 @@ struct Context {
     void free_mesh();
     void release_jit();
     linon_status upload_mesh(const float* xyz, uint32_t nv, const uint32_t* faces, uint32_t nf);
-    float viewport_height() const;
+    bool srgb_output = false;  // overlay_web epilogue
+    int num_cus = 0;
+
+    BatchState batch;
+    linon_batch_tuning tuning{16, 1, 32, 16};
+    linon_status batch_prepare(BatchState& B, const linon_view* views, int n);
+    linon_status batch_launch(BatchState& B, const linon_batch_tuning& T, hipStream_t s);
+    linon_status batch_capture();
+
+    void fill_scene(SceneParams& S) const;
     void fill_raycast_params(RaycastParams& P, const linon_integrator& ig) const;
     linon_status render_with(const linon_integrator& ig, hipStream_t s);
     hipEvent_t* next_timing_slot();
     linon_status read(const void* src, void* dst, size_t want, size_t bytes, const char* what);
 };
+
+void fill_step(StepParams& I, const linon_integrator& ig, float field_weight);
+void fill_camera(CameraParams& C, const linon_camera& cam, uint32_t w, uint32_t h);
 
 }  // namespace linon

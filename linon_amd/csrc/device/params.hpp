@@ -1,6 +1,6 @@
-// Launch parameters of the ray-cast and overlay kernels.  Plain C layout: the host fills
-// the struct and passes it by value, both to the statically compiled kernels and to kernels
-// compiled at run time with hiprtc (which receive it through hipModuleLaunchKernel).
+// Launch parameters of the ray-cast, batch and overlay kernels.  Plain C layout: the host fills
+// the structs and passes them by value (or, for per-view data of a batch, in a device array),
+// both to the statically compiled kernels and to kernels compiled at run time with hiprtc.
 #pragma once
 
 #define LINON_MODE_REF 0
@@ -18,39 +18,47 @@
 #define LINON_CNT_QUERIES 3
 #define LINON_CNT_RAYS 4
 #define LINON_CNT_SKIPS 5
+#define LINON_CNT_REFILLS 6
 #define LINON_NUM_COUNTERS 8
 
-struct RaycastParams {
-    // image
-    int width, height;
-    int tiles_x, tiles_y;
-    int sample_block;  // 1: block 0 traces the 8 sample rays of src/main_view.wgsl:377-480
-    int pad0;
-    // camera (per-ray generation as src/main_view.wgsl:347-366)
-    float origin[3];
-    float view_dir[3];
-    float up[3];
-    float viewport_h;  // 0.025 (reference) or 2*0.035*tan(fov/2)
-    // settings
-    float field_weight;
-    float mouse_x, mouse_y;
-    // integrator
-    float h_initial, t_max;
-    int adaptive, max_steps, mode, need_mapping;
-    float escape_r2;
-    // scene: per face three float4 (v0, v1-v0, v2-v0), w unused
-    const float* tris;
-    int num_faces;
-    int accel;
-    // uniform grid over the scene
+// The mesh and its uniform grid.
+struct SceneParams {
+    const float* tris;           // per face three float4: v0, v1 - v0, v2 - v0 (w unused)
     const unsigned* cell_start;  // ncells + 1 offsets into cell_faces
     const unsigned* cell_faces;  // face ids, ascending within a cell
     const unsigned char* dist;   // Chebyshev distance (cells) to the nearest occupied cell, capped
+    int num_faces;
+    int accel;
     float grid_lo[3];
     float grid_hi[3];
     float cell_size, inv_cell_size;
     float skip_margin;
     int gdim[3];
+};
+
+// The integration loop of one view (src/main_view.wgsl:229-279 plus the escape mode).
+struct StepParams {
+    float h_initial, t_max, field_weight, escape_r2;
+    int adaptive, max_steps, mode, need_mapping;
+};
+
+// One pinhole view (src/main_view.wgsl:347-366): CameraUniform's vectors and the viewport.
+struct CameraParams {
+    int width, height;
+    float origin[3];
+    float view_dir[3];
+    float up[3];
+    float viewport_h;  // 0.025 (reference) or 2 * 0.035 * tan(fov / 2)
+};
+
+struct RaycastParams {
+    CameraParams cam;
+    StepParams step;
+    SceneParams scene;
+    int tiles_x, tiles_y;
+    int sample_block;  // 1: block 0 traces the 8 sample rays of src/main_view.wgsl:377-480
+    int pad0;
+    float mouse_x, mouse_y;
     // outputs
     unsigned* color;      // rgba8 per texel
     float* mapping;       // float4 per texel
@@ -62,11 +70,36 @@ struct RaycastParams {
     int samples_per_ray, sample_steps;
 };
 
+// One view of a batch.  Its rays occupy [first_ray, first_ray + 64 * tiles) of the batch's
+// flattened ray index space, in 8x8-tile order, so consecutive indices are neighbours.
+struct BatchView {
+    CameraParams cam;
+    StepParams step;
+    unsigned long long first_ray;
+    int tiles_x, tiles;
+    unsigned* color;
+    float* mapping;
+    int* hit_face;
+    int* steps;
+};
+
+struct BatchParams {
+    const BatchView* views;
+    unsigned long long* queue;     // next unclaimed flattened ray index (atomic)
+    unsigned long long* counters;  // LINON_NUM_COUNTERS
+    unsigned long long total_rays;
+    int num_views;
+    int compact_every;  // RK4 iterations a wave runs between refills
+    int refill_min;     // refill once at least this many lanes are idle
+    int pad;
+    SceneParams scene;
+};
+
 struct OverlayParams {
     int width, height;
     int overlay_mode, delta;
     float scaling;
-    int pad;
+    int srgb;  // 1: linear -> sRGB encode of the result (overlay_web, src/overlay.wgsl:148-165)
     const unsigned* ray_casting;
     const float* mapping;
     unsigned* final_color;

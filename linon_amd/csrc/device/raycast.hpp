@@ -73,7 +73,7 @@ struct QueryStats {
 // distance to the boundary of the empty (2k-1)^3 block of cells around p's cell, where k is
 // the cell's Chebyshev distance to the nearest occupied cell; outside the grid, the distance
 // to the grid box (all faces lie inside it).
-__device__ __forceinline__ float grid_safe_distance(const RaycastParams& P, vec3 p) {
+__device__ __forceinline__ float grid_safe_distance(const SceneParams& P, vec3 p) {
     const float ics = P.inv_cell_size;
     float gx = (p.x - P.grid_lo[0]) * ics;
     float gy = (p.y - P.grid_lo[1]) * ics;
@@ -96,7 +96,7 @@ __device__ __forceinline__ float grid_safe_distance(const RaycastParams& P, vec3
     return fminf(fminf(bx, by), bz) * P.cell_size;
 }
 
-__device__ __forceinline__ void test_cell(const RaycastParams& P, int cell, vec3 o, vec3 d, float L,
+__device__ __forceinline__ void test_cell(const SceneParams& P, int cell, vec3 o, vec3 d, float L,
                                           float& best_t, int& best_face, QueryStats& st) {
     unsigned b = P.cell_start[cell], e = P.cell_start[cell + 1];
     st.cells += 1;
@@ -112,7 +112,7 @@ __device__ __forceinline__ void test_cell(const RaycastParams& P, int cell, vec3
 // takes exactly |dx|+|dy|+|dz| steps from the entry cell to the exit cell and only steps
 // along axes that still differ, so it always stays inside the grid and always ends; cells
 // grazed within rounding error are covered by the margin the cell lists are built with.
-__device__ __forceinline__ void query_grid(const RaycastParams& P, vec3 o, vec3 d, float L, float& best_t,
+__device__ __forceinline__ void query_grid(const SceneParams& P, vec3 o, vec3 d, float L, float& best_t,
                                            int& best_face, QueryStats& st) {
     const float ics = P.inv_cell_size;
     const float ax = (o.x - P.grid_lo[0]) * ics, ay = (o.y - P.grid_lo[1]) * ics, az = (o.z - P.grid_lo[2]) * ics;
@@ -204,7 +204,7 @@ struct CameraFrame {
     vec3 origin, vd, nh, nv;
     float vw, vh;
 };
-__device__ __forceinline__ CameraFrame camera_frame(const RaycastParams& P) {
+__device__ __forceinline__ CameraFrame camera_frame(const CameraParams& P) {
     CameraFrame c;
     c.origin = ld3(P.origin);
     c.vd = ld3(P.view_dir);
@@ -220,7 +220,7 @@ __device__ __forceinline__ CameraFrame camera_frame(const RaycastParams& P) {
     c.nv = normalize(vertical);
     return c;
 }
-__device__ __forceinline__ vec3 camera_dir(const CameraFrame& c, const RaycastParams& P, int gx, int gy) {
+__device__ __forceinline__ vec3 camera_dir(const CameraFrame& c, const CameraParams& P, int gx, int gy) {
     float width = (float)P.width, height = (float)P.height;
     float u = (float)gx / (width - 1.0f) * c.vw - 0.5f * c.vw;
     float v = (float)gy / (height - 1.0f) * c.vh - 0.5f * c.vh;
@@ -265,7 +265,7 @@ __device__ void sample_rays(const RaycastParams& P, const CameraFrame& cam, int 
     vec3 s2 = u2 * cam.nh + v2 * cam.nv + 0.035f * cam.vd;
     const vec3 start_dir = normalize(s2);
     vec3 p = cam.origin, d = start_dir;
-    const float w = P.field_weight;
+    const float w = P.step.field_weight;
     const float h = 0.001f, hh = 0.5f * h;
     float t = 0.0f;
     const int nsamp = P.samples_per_ray;
@@ -289,11 +289,171 @@ __device__ void sample_rays(const RaycastParams& P, const CameraFrame& cam, int 
     }
 }
 
+// One ray's integration state (src/main_view.wgsl:233-242), kept in registers.
+struct RayState {
+    vec3 p, d, v0, last_v;
+    float t, h, last_diff;
+    float safe;  // grid path: lower bound on the distance from p to the mesh
+    float cr, cg, cb;
+    int face, iters;
+    bool has_color;
+    QueryStats st;
+};
+
+__device__ __forceinline__ void ray_begin(RayState& r, vec3 origin, vec3 dir, const StepParams& I) {
+    r.p = origin;
+    r.d = dir;
+    r.v0 = dir;
+    r.last_v = vec3f(0.0f, 0.0f, 0.0f);
+    r.t = 0.0f;
+    r.h = I.h_initial;
+    r.last_diff = -1.0f;
+    r.safe = 0.0f;
+    r.cr = r.cg = r.cb = 0.0f;
+    r.face = -1;
+    r.iters = 0;
+    r.has_color = false;
+    r.st = QueryStats{0ull, 0u, 0u, 0u};
+}
+
+__device__ __forceinline__ bool ray_finished(const RayState& r, const StepParams& I) {
+    return r.iters >= I.max_steps || (I.mode != LINON_MODE_ESCAPE && !(r.t <= I.t_max));
+}
+
+// Runs at most `budget` loop iterations of nonlinear_ray_color (src/main_view.wgsl:244-271):
+// RK4 with the previous stage passed as `v`, the adaptive redo at h = 0.001, the blend with
+// field_weight, the segment query until the first hit, and the escape tests of the escape
+// mode.  Returns true once the ray is finished.  `lds` holds the faces when the brute-force
+// path staged them, else null.
+template <class F>
+__device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, const SceneParams& S,
+                                            const float4* lds, int budget) {
+    const F field{};
+    const bool adaptive = I.adaptive != 0;
+    const bool escape = (I.mode == LINON_MODE_ESCAPE);
+    const bool brute = (S.accel == LINON_ACCEL_BRUTE_K);
+    const float margin = S.skip_margin;
+    const float w = I.field_weight;
+    const vec3 start_dir = r.v0;
+    for (int k = 0; k < budget; ++k) {
+        if (r.iters >= I.max_steps) return true;
+        if (!escape && !(r.t <= I.t_max)) return true;
+        ++r.iters;
+        const float h = r.h;
+        const float hh = 0.5f * h;
+        const vec3 p = r.p;
+        vec3 k1 = field(p, p, start_dir, r.d, r.t);
+        vec3 k2 = field(p, p + hh * k1, start_dir, k1, r.t + hh);
+        vec3 k3 = field(p, p + hh * k2, start_dir, k2, r.t + hh);
+        vec3 k4 = field(p, p + h * k3, start_dir, k3, r.t + h);
+        vec3 V = rk4_average(k1, k2, k3, k4);
+        float diff = 0.0f;
+        if (adaptive) {  // src/main_view.wgsl:251-255
+            diff = length(V - r.last_v);
+            if (r.last_diff >= 0.0f && diff > 10.0f * r.last_diff && h > 0.002f) {
+                r.h = 0.001f;
+                continue;
+            }
+        }
+        r.d = (1.0f - w) * r.d + w * V;
+        const vec3 step = r.d * h;
+        if (!r.has_color) {
+            const float L = length(step);
+            bool query = (L > 0.0f && L < __builtin_inff());  // normalize(0) or NaN: no hit
+            if (query && !brute) {
+                if (!(r.safe > L + margin)) r.safe = grid_safe_distance(S, p);
+                query = !(r.safe > L + margin);
+                r.st.skips += query ? 0u : 1u;
+                r.safe -= L;
+            }
+            if (query) {
+                r.st.queries += 1;
+                const vec3 nd = step / L;  // normalize(step_dir)
+                float best_t = -1.0f;
+                int best_face = -1;
+                if (brute) {
+                    const int nf = S.num_faces;
+                    r.st.tests += (unsigned long long)nf;
+                    if (lds) {
+                        for (int f = 0; f < nf; ++f) {
+                            float4 a = lds[3 * f], e1 = lds[3 * f + 1], e2 = lds[3 * f + 2];
+                            float tt = mt_test(vec3f(a.x, a.y, a.z), vec3f(e1.x, e1.y, e1.z),
+                                               vec3f(e2.x, e2.y, e2.z), p, nd);
+                            consider(tt, f, L, best_t, best_face);
+                        }
+                    } else {
+                        for (int f = 0; f < nf; ++f) {
+                            float tt = mt_test(ld_tri_vec(S.tris, f, 0), ld_tri_vec(S.tris, f, 1),
+                                               ld_tri_vec(S.tris, f, 2), p, nd);
+                            consider(tt, f, L, best_t, best_face);
+                        }
+                    }
+                } else {
+                    query_grid(S, p, nd, L, best_t, best_face, r.st);
+                }
+                if (best_face >= 0) {  // shading of ray_color, src/main_view.wgsl:199-218
+                    r.has_color = true;
+                    r.face = best_face;
+                    vec3 e1 = ld_tri_vec(S.tris, best_face, 1), e2 = ld_tri_vec(S.tris, best_face, 2);
+                    vec3 n = normalize(cross(e1, e2));
+                    float intensity = fmaxf(dot(n, -nd), 0.0f);
+                    if (intensity == 0.0f) intensity = fmaxf(dot(-n, -nd), 0.0f);
+                    float spec = powf(intensity, 64.0f);
+                    float s = (0.01f + intensity) + spec;
+                    r.cr = s * fabsf(n.x);
+                    r.cg = s * fabsf(n.y);
+                    r.cb = s * fabsf(n.z);
+                }
+            }
+        }
+        r.p = p + step;
+        r.t = r.t + h;
+        if (adaptive) {  // src/main_view.wgsl:266-270
+            r.last_v = V;
+            if (2.0f * diff < r.last_diff && h < I.h_initial) r.h = I.h_initial;
+            r.last_diff = diff;
+        }
+        if (escape) {
+            if (r.has_color && !I.need_mapping) return true;
+            if (dot(r.p, r.p) > I.escape_r2 && dot(r.p, r.d) > 0.0f) return true;
+        }
+    }
+    return ray_finished(r, I);
+}
+
+// Colour (alpha forced to 1, black when nothing was hit) and end point of a finished ray
+// (src/main_view.wgsl:273-277, 372-374).
+__device__ __forceinline__ void ray_store(const RayState& r, size_t texel, unsigned* color, float* mapping,
+                                          int* hit_face, int* steps) {
+    color[texel] = pack_rgba8(r.cr, r.cg, r.cb, 1.0f);
+    reinterpret_cast<float4*>(mapping)[texel] = make_float4(r.p.x, r.p.y, r.p.z, 1.0f);
+    if (hit_face) hit_face[texel] = r.face;
+    if (steps) steps[texel] = r.iters;
+}
+
+__device__ __forceinline__ void count_wave(unsigned long long* counters, int lane, unsigned long long steps,
+                                           const QueryStats& st, unsigned long long rays, unsigned long long refills) {
+    unsigned long long c_steps = wave_sum(steps);
+    unsigned long long c_tests = wave_sum(st.tests);
+    unsigned long long c_cells = wave_sum((unsigned long long)st.cells);
+    unsigned long long c_q = wave_sum((unsigned long long)st.queries);
+    unsigned long long c_rays = wave_sum(rays);
+    unsigned long long c_skip = wave_sum((unsigned long long)st.skips);
+    if (counters && lane == 0) {
+        atomicAdd(counters + LINON_CNT_STEPS, c_steps);
+        atomicAdd(counters + LINON_CNT_TESTS, c_tests);
+        atomicAdd(counters + LINON_CNT_CELLS, c_cells);
+        atomicAdd(counters + LINON_CNT_QUERIES, c_q);
+        atomicAdd(counters + LINON_CNT_RAYS, c_rays);
+        atomicAdd(counters + LINON_CNT_SKIPS, c_skip);
+        if (refills) atomicAdd(counters + LINON_CNT_REFILLS, refills);
+    }
+}
+
 template <class F>
 __device__ __forceinline__ void raycast_body(const RaycastParams& P) {
     extern __shared__ float4 lds_tris[];
-    const F field{};
-    const CameraFrame cam = camera_frame(P);
+    const CameraFrame cam = camera_frame(P.cam);
     int b = blockIdx.x;
     const int lane = threadIdx.x;
     if (P.sample_block) {
@@ -308,140 +468,23 @@ __device__ __forceinline__ void raycast_body(const RaycastParams& P) {
     const int gx = (tile % P.tiles_x) * LINON_TILE + (lane & (LINON_TILE - 1));
     const int gy = (tile / P.tiles_x) * LINON_TILE + (lane / LINON_TILE);
 
-    const bool brute = (P.accel == LINON_ACCEL_BRUTE_K);
-    const bool brute_lds = brute && P.num_faces <= LINON_MAX_BRUTE_LDS;
+    const bool brute_lds = P.scene.accel == LINON_ACCEL_BRUTE_K && P.scene.num_faces <= LINON_MAX_BRUTE_LDS;
     if (brute_lds) {  // stage the faces once per wave; every lane then reads them from LDS
-        const float4* src = reinterpret_cast<const float4*>(P.tris);
-        for (int i = lane; i < 3 * P.num_faces; i += blockDim.x) lds_tris[i] = src[i];
+        const float4* src = reinterpret_cast<const float4*>(P.scene.tris);
+        for (int i = lane; i < 3 * P.scene.num_faces; i += blockDim.x) lds_tris[i] = src[i];
         __syncthreads();
     }
     // lanes outside the image stay in the wave (zero work) so the wave reductions below see
     // every lane
-    const bool active = gx < P.width && gy < P.height;
+    const bool active = gx < P.cam.width && gy < P.cam.height;
 
-    const vec3 start_dir = camera_dir(cam, P, gx, gy);
-    vec3 p = cam.origin;
-    vec3 d = start_dir;
-    const float w = P.field_weight;
-    const float h_initial = P.h_initial;
-    float h = h_initial;
-    float t = 0.0f;
-    vec3 last_v = vec3f(0.0f, 0.0f, 0.0f);
-    float last_diff = -1.0f;
-    bool has_color = false;
-    float cr = 0.0f, cg = 0.0f, cb = 0.0f;
-    int face = -1;
-    float safe = 0.0f;  // grid path: lower bound on the distance from p to the mesh
-    QueryStats st{0ull, 0u, 0u, 0u};
-    int iters = 0;
-    const bool adaptive = P.adaptive != 0;
-    const bool escape = (P.mode == LINON_MODE_ESCAPE);
-    const float margin = P.skip_margin;
-
-    while (active && iters < P.max_steps) {
-        if (!escape && !(t <= P.t_max)) break;
-        ++iters;
-        const float hh = 0.5f * h;
-        vec3 k1 = field(p, p, start_dir, d, t);
-        vec3 k2 = field(p, p + hh * k1, start_dir, k1, t + hh);
-        vec3 k3 = field(p, p + hh * k2, start_dir, k2, t + hh);
-        vec3 k4 = field(p, p + h * k3, start_dir, k3, t + h);
-        vec3 V = rk4_average(k1, k2, k3, k4);
-        float diff = 0.0f;
-        if (adaptive) {  // src/main_view.wgsl:251-255
-            diff = length(V - last_v);
-            if (last_diff >= 0.0f && diff > 10.0f * last_diff && h > 0.002f) {
-                h = 0.001f;
-                continue;
-            }
-        }
-        d = (1.0f - w) * d + w * V;
-        const vec3 step = d * h;
-        if (!has_color) {
-            const float L = length(step);
-            bool query = (L > 0.0f && L < __builtin_inff());  // normalize(0) or NaN: no hit
-            if (query && !brute) {
-                if (!(safe > L + margin)) safe = grid_safe_distance(P, p);
-                query = !(safe > L + margin);
-                st.skips += query ? 0u : 1u;
-                safe -= L;
-            }
-            if (query) {
-                st.queries += 1;
-                const vec3 nd = step / L;  // normalize(step_dir)
-                float best_t = -1.0f;
-                int best_face = -1;
-                if (brute) {
-                    const int nf = P.num_faces;
-                    st.tests += (unsigned long long)nf;
-                    if (brute_lds) {
-                        for (int f = 0; f < nf; ++f) {
-                            float4 a = lds_tris[3 * f], e1 = lds_tris[3 * f + 1], e2 = lds_tris[3 * f + 2];
-                            float tt = mt_test(vec3f(a.x, a.y, a.z), vec3f(e1.x, e1.y, e1.z),
-                                               vec3f(e2.x, e2.y, e2.z), p, nd);
-                            consider(tt, f, L, best_t, best_face);
-                        }
-                    } else {
-                        for (int f = 0; f < nf; ++f) {
-                            float tt = mt_test(ld_tri_vec(P.tris, f, 0), ld_tri_vec(P.tris, f, 1),
-                                               ld_tri_vec(P.tris, f, 2), p, nd);
-                            consider(tt, f, L, best_t, best_face);
-                        }
-                    }
-                } else {
-                    query_grid(P, p, nd, L, best_t, best_face, st);
-                }
-                if (best_face >= 0) {  // shading of ray_color, src/main_view.wgsl:199-218
-                    has_color = true;
-                    face = best_face;
-                    vec3 e1 = ld_tri_vec(P.tris, best_face, 1), e2 = ld_tri_vec(P.tris, best_face, 2);
-                    vec3 n = normalize(cross(e1, e2));
-                    float intensity = fmaxf(dot(n, -nd), 0.0f);
-                    if (intensity == 0.0f) intensity = fmaxf(dot(-n, -nd), 0.0f);
-                    float spec = powf(intensity, 64.0f);
-                    float s = (0.01f + intensity) + spec;
-                    cr = s * fabsf(n.x);
-                    cg = s * fabsf(n.y);
-                    cb = s * fabsf(n.z);
-                }
-            }
-        }
-        p = p + step;
-        t = t + h;
-        if (adaptive) {  // src/main_view.wgsl:266-270
-            last_v = V;
-            if (2.0f * diff < last_diff && h < h_initial) h = h_initial;
-            last_diff = diff;
-        }
-        if (escape) {
-            if (has_color && !P.need_mapping) break;
-            if (dot(p, p) > P.escape_r2 && dot(p, d) > 0.0f) break;
-        }
-    }
-
+    RayState r;
+    ray_begin(r, cam.origin, camera_dir(cam, P.cam, gx, gy), P.step);
     if (active) {
-        const size_t texel = (size_t)(P.height - 1 - gy) * P.width + gx;
-        P.color[texel] = pack_rgba8(cr, cg, cb, 1.0f);
-        reinterpret_cast<float4*>(P.mapping)[texel] = make_float4(p.x, p.y, p.z, 1.0f);
-        if (P.hit_face) P.hit_face[texel] = face;
-        if (P.steps) P.steps[texel] = iters;
+        ray_advance<F>(r, P.step, P.scene, brute_lds ? lds_tris : nullptr, 0x7fffffff);
+        ray_store(r, (size_t)(P.cam.height - 1 - gy) * P.cam.width + gx, P.color, P.mapping, P.hit_face, P.steps);
     }
-
-    // per-wave counters, one atomic per counter per wave
-    unsigned long long c_steps = wave_sum((unsigned long long)iters);
-    unsigned long long c_tests = wave_sum(st.tests);
-    unsigned long long c_cells = wave_sum((unsigned long long)st.cells);
-    unsigned long long c_q = wave_sum((unsigned long long)st.queries);
-    unsigned long long c_rays = wave_sum(active ? 1ull : 0ull);
-    unsigned long long c_skip = wave_sum((unsigned long long)st.skips);
-    if (P.counters && lane == 0) {
-        atomicAdd(P.counters + LINON_CNT_STEPS, c_steps);
-        atomicAdd(P.counters + LINON_CNT_TESTS, c_tests);
-        atomicAdd(P.counters + LINON_CNT_CELLS, c_cells);
-        atomicAdd(P.counters + LINON_CNT_QUERIES, c_q);
-        atomicAdd(P.counters + LINON_CNT_RAYS, c_rays);
-        atomicAdd(P.counters + LINON_CNT_SKIPS, c_skip);
-    }
+    count_wave(P.counters, lane, (unsigned long long)r.iters, r.st, active ? 1ull : 0ull, 0ull);
 }
 
 // src/overlay.wgsl:99-146, one lane per texel.
@@ -467,13 +510,20 @@ __device__ __forceinline__ float lyapunov_exponent(const float4* m, int W, int x
     return __builtin_sqrtf(fmaxf(e0, e1));
 }
 
+__device__ __forceinline__ float srgb_from_linear(float x) {  // src/overlay.wgsl:148-154
+    return x < 0.0031308f ? x / 12.92f : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f;
+}
+
 __device__ __forceinline__ void overlay_body(const OverlayParams& P) {
     const int x = blockIdx.x * 16 + (threadIdx.x & 15);
     const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= P.width || y >= P.height) return;
     const size_t idx = (size_t)y * P.width + x;
     const unsigned c = P.ray_casting[idx];
-    unsigned outc = c;
+    // textureLoad of the rgba8unorm ray-cast texture
+    float col[4];
+    for (int k = 0; k < 4; ++k) col[k] = (float)((c >> (8 * k)) & 255u) * (1.0f / 255.0f);
+    bool blended = false;
     float lam = -1.0f;
     const int pad = 2 * P.delta;
     if (P.overlay_mode == 1 && !(x < pad || y < pad || x >= P.width - pad || y >= P.height - pad)) {
@@ -483,11 +533,15 @@ __device__ __forceinline__ void overlay_body(const OverlayParams& P) {
         if (eidx < (size_t)P.width * P.height) P.exponents[eidx] = s;
         if (s >= 0.8f) {
             float alpha = fminf(1.0f, s) * 0.5f;
-            float r = (float)(c & 255u) * (1.0f / 255.0f), g = (float)((c >> 8) & 255u) * (1.0f / 255.0f);
-            float b = (float)((c >> 16) & 255u) * (1.0f / 255.0f), a = (float)(c >> 24) * (1.0f / 255.0f);
-            outc = pack_rgba8(alpha + (1.0f - alpha) * r, alpha + (1.0f - alpha) * g,
-                              alpha + (1.0f - alpha) * b, alpha + (1.0f - alpha) * a);
+            for (int k = 0; k < 4; ++k) col[k] = alpha + (1.0f - alpha) * col[k];
+            blended = true;
         }
+    }
+    unsigned outc = c;
+    if (P.srgb) {  // overlay_web stores vec4(srgb_from_linear(color.rgb), color.a)
+        outc = pack_rgba8(srgb_from_linear(col[0]), srgb_from_linear(col[1]), srgb_from_linear(col[2]), col[3]);
+    } else if (blended) {
+        outc = pack_rgba8(col[0], col[1], col[2], col[3]);
     }
     P.final_color[idx] = outc;
     if (P.lyapunov) P.lyapunov[idx] = lam;

@@ -3,6 +3,7 @@
 // so that a caller may capture linon_render into a HIP graph).
 #include <hip/hip_runtime.h>
 
+#include "device/batch.hpp"
 #include "device/fields.hpp"
 #include "device/raycast.hpp"
 #include "kernels.hpp"
@@ -14,6 +15,11 @@ __global__ void __launch_bounds__(64) raycast_kernel(RaycastParams P) {
     lk::raycast_body<F>(P);
 }
 
+template <class F>
+__global__ void __launch_bounds__(256) batch_kernel(BatchParams B) {
+    lk::batch_body<F>(B);
+}
+
 __global__ void __launch_bounds__(256) overlay_kernel(OverlayParams P) { lk::overlay_body(P); }
 
 __global__ void __launch_bounds__(64) clear_u64_kernel(unsigned long long* p, int n) {
@@ -22,6 +28,34 @@ __global__ void __launch_bounds__(64) clear_u64_kernel(unsigned long long* p, in
 }
 
 bool field_is_predefined(int id) { return (id >= 1 && id <= 9) || (id >= 16 && id <= 20); }
+
+#define LINON_FIELD_TABLE(KERNEL)                                      \
+    switch (id) {                                                      \
+        case 1: return (const void*)&KERNEL<lw::FieldMirageSpherical>; \
+        case 2: return (const void*)&KERNEL<lw::FieldMiragePlane>;     \
+        case 3: return (const void*)&KERNEL<lw::FieldMirageSphericalSigmoid>; \
+        case 4: return (const void*)&KERNEL<lw::FieldMiragePlaneSigmoid>; \
+        case 5: return (const void*)&KERNEL<lw::FieldTranslationX>;    \
+        case 6: return (const void*)&KERNEL<lw::FieldTranslationZ>;    \
+        case 7: return (const void*)&KERNEL<lw::FieldRotation>;        \
+        case 8: return (const void*)&KERNEL<lw::FieldLorenz>;          \
+        case 9: return (const void*)&KERNEL<lw::FieldRoessler>;        \
+        case 16: return (const void*)&KERNEL<lw::FieldIdentity>;       \
+        case 17: return (const void*)&KERNEL<lw::FieldSwirl>;          \
+        case 18: return (const void*)&KERNEL<lw::FieldLens>;           \
+        case 19: return (const void*)&KERNEL<lw::FieldHotAir>;         \
+        case 20: return (const void*)&KERNEL<lw::FieldWarp>;           \
+        default: return nullptr;                                       \
+    }
+
+const void* batch_kernel_for(int id) { LINON_FIELD_TABLE(batch_kernel) }
+
+hipError_t launch_batch(int field, const BatchParams& B, unsigned blocks, unsigned threads, hipStream_t s) {
+    const void* fn = batch_kernel_for(field);
+    if (!fn) return hipErrorInvalidDeviceFunction;
+    void* args[] = {const_cast<BatchParams*>(&B)};
+    return hipLaunchKernel(fn, dim3(blocks), dim3(threads), args, 0, s);
+}
 
 const void* raycast_kernel_for(int id) {
     switch (id) {

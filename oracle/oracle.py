@@ -57,6 +57,8 @@ class IntegratorIn(C.Structure):
 
 
 _lib = None
+_lib_fma = None
+LIB_FMA_PATH = HERE / "liblinon_oracle_fma.so"
 
 
 def build() -> Path:
@@ -64,40 +66,52 @@ def build() -> Path:
     return LIB_PATH
 
 
-def lib() -> C.CDLL:
-    global _lib
+def lib(fma: bool = False) -> C.CDLL:
+    """The oracle library; fma=True gives the build with FMA contraction allowed (a second
+    FP32 rounding path of the same restatement)."""
+    global _lib, _lib_fma
+    if fma:
+        if _lib_fma is None:
+            if not LIB_FMA_PATH.exists():
+                build()
+            _lib_fma = _bind(C.CDLL(str(LIB_FMA_PATH)))
+        return _lib_fma
     if _lib is None:
         if not LIB_PATH.exists():
             build()
-        _lib = C.CDLL(str(LIB_PATH))
-        P = C.c_void_p
-        _lib.oracle_render_f32.argtypes = [P, P, P, C.c_int, P, C.c_uint32, P, C.c_uint32, C.c_int, C.c_int,
-                                           C.c_int, C.c_int, C.c_int, P, P, P, P, P, P, P]
-        _lib.oracle_render_f64.argtypes = _lib.oracle_render_f32.argtypes
-        _lib.oracle_set_prefilter.argtypes = [C.c_int]
-        _lib.oracle_set_prefilter.restype = None
-        _lib.oracle_render_pixels_f32.argtypes = [P, P, P, C.c_int, P, C.c_uint32, P, C.c_uint32, C.c_int, C.c_int,
-                                                  P, C.c_int, C.c_int, P, P, P, P]
-        _lib.oracle_render_pixels_f64.argtypes = _lib.oracle_render_pixels_f32.argtypes
-        _lib.oracle_overlay_f32.argtypes = [P, C.c_int, C.c_int, P, P, P, P, P]
-        _lib.oracle_overlay_f64.argtypes = _lib.oracle_overlay_f32.argtypes
-        _lib.oracle_sample_rays_f32.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int, P]
-        _lib.oracle_sample_rays_f64.argtypes = _lib.oracle_sample_rays_f32.argtypes
-        _lib.oracle_hit_triangle_f32.argtypes = [P, P, P]
-        _lib.oracle_hit_triangle_f32.restype = C.c_float
-        _lib.oracle_hit_triangle_f64.argtypes = [P, P, P]
-        _lib.oracle_hit_triangle_f64.restype = C.c_double
-        _lib.oracle_ray_color_f32.argtypes = [P, C.c_uint32, P, C.c_uint32, P, P, C.c_float, P]
-        _lib.oracle_field_f64.argtypes = [C.c_int, P, P, P, P, C.c_double, P]
-        _lib.oracle_field_f32.argtypes = [C.c_int, P, P, P, P, C.c_float, P]
-        _lib.oracle_refraction_index_f64.argtypes = [C.c_double]
-        _lib.oracle_refraction_index_f64.restype = C.c_double
-        _lib.oracle_refraction_f64.argtypes = [C.c_double, C.c_double, P, P, P]
-        _lib.oracle_lyapunov_f64.argtypes = [P, P, P, P]
-        _lib.oracle_lyapunov_f64.restype = C.c_double
-        _lib.oracle_camera_ray_f64.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, P]
-        _lib.oracle_trace_f64.argtypes = [C.c_int, P, C.c_double, P, C.c_uint32, P, C.c_uint32, P, P, P, P]
-        _lib.oracle_trace_f32.argtypes = [C.c_int, P, C.c_float, P, C.c_uint32, P, C.c_uint32, P, P, P, P]
+        _lib = _bind(C.CDLL(str(LIB_PATH)))
+    return _lib
+
+
+def _bind(_lib: C.CDLL) -> C.CDLL:
+    P = C.c_void_p
+    _lib.oracle_render_f32.argtypes = [P, P, P, C.c_int, P, C.c_uint32, P, C.c_uint32, C.c_int, C.c_int,
+                                       C.c_int, C.c_int, C.c_int, P, P, P, P, P, P, P]
+    _lib.oracle_render_f64.argtypes = _lib.oracle_render_f32.argtypes
+    _lib.oracle_set_prefilter.argtypes = [C.c_int]
+    _lib.oracle_set_prefilter.restype = None
+    _lib.oracle_render_pixels_f32.argtypes = [P, P, P, C.c_int, P, C.c_uint32, P, C.c_uint32, C.c_int, C.c_int,
+                                              P, C.c_int, C.c_int, P, P, P, P]
+    _lib.oracle_render_pixels_f64.argtypes = _lib.oracle_render_pixels_f32.argtypes
+    _lib.oracle_overlay_f32.argtypes = [P, C.c_int, C.c_int, P, P, P, P, P]
+    _lib.oracle_overlay_f64.argtypes = _lib.oracle_overlay_f32.argtypes
+    _lib.oracle_sample_rays_f32.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int, P]
+    _lib.oracle_sample_rays_f64.argtypes = _lib.oracle_sample_rays_f32.argtypes
+    _lib.oracle_hit_triangle_f32.argtypes = [P, P, P]
+    _lib.oracle_hit_triangle_f32.restype = C.c_float
+    _lib.oracle_hit_triangle_f64.argtypes = [P, P, P]
+    _lib.oracle_hit_triangle_f64.restype = C.c_double
+    _lib.oracle_ray_color_f32.argtypes = [P, C.c_uint32, P, C.c_uint32, P, P, C.c_float, P]
+    _lib.oracle_field_f64.argtypes = [C.c_int, P, P, P, P, C.c_double, P]
+    _lib.oracle_field_f32.argtypes = [C.c_int, P, P, P, P, C.c_float, P]
+    _lib.oracle_refraction_index_f64.argtypes = [C.c_double]
+    _lib.oracle_refraction_index_f64.restype = C.c_double
+    _lib.oracle_refraction_f64.argtypes = [C.c_double, C.c_double, P, P, P]
+    _lib.oracle_lyapunov_f64.argtypes = [P, P, P, P]
+    _lib.oracle_lyapunov_f64.restype = C.c_double
+    _lib.oracle_camera_ray_f64.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, P]
+    _lib.oracle_trace_f64.argtypes = [C.c_int, P, C.c_double, P, C.c_uint32, P, C.c_uint32, P, P, P, P]
+    _lib.oracle_trace_f32.argtypes = [C.c_int, P, C.c_float, P, C.c_uint32, P, C.c_uint32, P, P, P, P]
     return _lib
 
 
@@ -146,10 +160,10 @@ def render(camera, settings, integ, field: int, verts, faces, W: int, H: int, ro
            precision: str = "f32") -> dict:
     """Ray-cast pass; returns texel-space arrays (row 0 at the top).  ``rows`` = (gy0, gy1)
     restricts the invocation rows (gy counts from the image bottom)."""
-    L = lib()
+    L = lib(fma=precision == "f32fma")
     v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
     f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
-    ft = np.float32 if precision == "f32" else np.float64
+    ft = np.float64 if precision == "f64" else np.float32
     color8 = np.zeros((H, W, 4), np.uint8)
     color = np.zeros((H, W, 4), ft)
     mapping = np.zeros((H, W, 4), ft)
@@ -158,7 +172,7 @@ def render(camera, settings, integ, field: int, verts, faces, W: int, H: int, ro
     counters = np.zeros(2, np.int64)
     flags = np.zeros((H, W), np.uint8)
     r0, r1 = rows if rows is not None else (0, H)
-    fn = L.oracle_render_f32 if precision == "f32" else L.oracle_render_f64
+    fn = L.oracle_render_f64 if precision == "f64" else L.oracle_render_f32
     rc = fn(C.byref(camera), C.byref(settings), C.byref(integ), field, _p(v), len(v), _p(f), len(f), W, H, r0, r1,
             threads, _p(color8), _p(color), _p(mapping), _p(hit), _p(steps), _p(counters), _p(flags))
     if rc != 0:
@@ -170,16 +184,16 @@ def render(camera, settings, integ, field: int, verts, faces, W: int, H: int, ro
 def render_pixels(camera, settings, integ, field: int, verts, faces, W: int, H: int, pixels, threads: int = 0,
                   precision: str = "f32"):
     """Rays of an explicit (n, 2) list of invocation coordinates (gx, gy)."""
-    L = lib()
+    L = lib(fma=precision == "f32fma")
     v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
     f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
     px = np.ascontiguousarray(pixels, dtype=np.int32).reshape(-1, 2)
     n = len(px)
-    mapping = np.zeros((n, 4), np.float32 if precision == "f32" else np.float64)
+    mapping = np.zeros((n, 4), np.float64 if precision == "f64" else np.float32)
     hit = np.zeros(n, np.int32)
     steps = np.zeros(n, np.int32)
     counters = np.zeros(2, np.int64)
-    fn = L.oracle_render_pixels_f32 if precision == "f32" else L.oracle_render_pixels_f64
+    fn = L.oracle_render_pixels_f64 if precision == "f64" else L.oracle_render_pixels_f32
     rc = fn(C.byref(camera), C.byref(settings), C.byref(integ), field, _p(v), len(v), _p(f),
             len(f), W, H, _p(px), n, threads, _p(mapping), _p(hit), _p(steps), _p(counters))
     if rc != 0:

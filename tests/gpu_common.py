@@ -47,9 +47,13 @@ def run_oracle(cam, st, ig, fid, verts, faces, W, H, precision="f32"):
 
 
 def run_oracles(cam, st, ig, fid, verts, faces, W, H):
-    """FP32 restatement (the parity oracle) and its FP64 instantiation (the accuracy reference)."""
-    return (run_oracle(cam, st, ig, fid, verts, faces, W, H, "f32"),
-            run_oracle(cam, st, ig, fid, verts, faces, W, H, "f64"))
+    """FP32 restatement (the parity oracle), the same restatement built with FMA contraction
+    (a second valid FP32 rounding path) and the FP64 instantiation (the accuracy reference)."""
+    r32 = run_oracle(cam, st, ig, fid, verts, faces, W, H, "f32")
+    r64 = run_oracle(cam, st, ig, fid, verts, faces, W, H, "f64")
+    rf = run_oracle(cam, st, ig, fid, verts, faces, W, H, "f32fma")
+    r64["mapping_f32fma"] = rf["mapping"]
+    return r32, r64
 
 
 def rel_err(a, b):
@@ -68,13 +72,16 @@ def low_lambda_mask(mapping, delta=1):
     return (lam >= 0) & (lam < LAMBDA_LOW), lam
 
 
-# A pixel is well conditioned when the FP32 restatement's own exit point is within WELL_COND of
-# the FP64 one; there the GPU must match the FP32 oracle to EXIT_RTOL.  Elsewhere FP32
-# rounding alone moves the exit point by more than the tolerance, and the GPU must be as
-# accurate as the FP32 restatement: its error against FP64 at most ILL_FACTOR times the
-# FP32 restatement's (and never required below EXIT_RTOL).
+# A pixel is well conditioned when both FP32 restatements (without and with FMA contraction)
+# end within WELL_COND of the FP64 end point; there the GPU must match the FP32 oracle to
+# EXIT_RTOL.  Elsewhere FP32 rounding alone moves the end point by more than the tolerance
+# (e.g. the cancellation in refraction's `eta * cosi - sqrt(k)` near grazing incidence), and
+# the GPU must be as accurate as the FP32 restatements: per pixel its error against FP64 at
+# most ILL_FACTOR times theirs (never required below EXIT_RTOL), and its 99th-percentile error
+# over the gated pixels at most Q99_FACTOR times theirs.
 WELL_COND = 1e-5
 ILL_FACTOR = 4.0
+Q99_FACTOR = 2.0
 
 
 def compare(gpu: "L.Frame", ref: dict, ref64: dict, label: str, lam_gpu=None):
@@ -91,6 +98,8 @@ def compare(gpu: "L.Frame", ref: dict, ref64: dict, label: str, lam_gpu=None):
     both = low & noisy
     noisy_hit_agree = float((gpu.hit_faces[both] == ref["hit"][both]).mean()) if both.any() else 1.0
     e32 = rel_err(ref["mapping"], ref64["mapping"])
+    if "mapping_f32fma" in ref64:
+        e32 = np.maximum(e32, rel_err(ref64["mapping_f32fma"], ref64["mapping"]))
     eg = rel_err(gpu.mapping, ref64["mapping"])
     d = rel_err(gpu.mapping, ref["mapping"])
     well = e32 <= WELL_COND
@@ -105,5 +114,6 @@ def compare(gpu: "L.Frame", ref: dict, ref64: dict, label: str, lam_gpu=None):
                 well_frac=float(well[mask].mean()) if n else 0.0, exit_ok=float(ok[mask].mean()) if n else 1.0,
                 exit_rel_vs_f32_well_max=float(d[mask & well].max()) if (mask & well).any() else 0.0,
                 err_f32_vs_f64_q50_q99_max=q(e32), err_gpu_vs_f64_q50_q99_max=q(eg), color_maxdiff=int(cdiff))
+    info["q99_ok"] = (not n) or q(eg)[1] <= max(Q99_FACTOR * q(e32)[1], EXIT_RTOL)
     print(info, flush=True)
     return info

@@ -1,0 +1,108 @@
+"""GPU: the persistent multi-view batch kernel (work queue + wave-level ray compaction),
+its hipGraph replay and the launch-shape autotuner."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import linon_amd as L
+from linon_amd import configs, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup():
+    v, f = scenes.benchmark_scene(level=3)
+    view = L.MainView(64, 64)
+    view.update_model(v, f)
+    view.set_integrator(configs.batch_integrator(max_steps=3000))
+    views = configs.batch_views(num_views=9, seed=3, scale=16)
+    yield view, views
+    view.close()
+
+
+def _plain(view, bv):
+    view.resize_texture(bv.width, bv.height)
+    view.update_camera(bv.camera)
+    view.update_settings(L.Settings.make(field_weight=bv.field_weight))
+    ig = configs.batch_integrator(max_steps=3000)
+    ig.h_initial = bv.h_initial
+    view.set_integrator(ig)
+    view.set_field(bv.field)
+    view.render()
+    return view.read_frame()
+
+
+def test_batch_views_equal_the_plain_kernel(setup):
+    """Every view of a batch (mixed fields, step sizes and resolutions, escape mode) is bit for
+    bit the frame the plain kernel renders for that view alone."""
+    view, views = setup
+    print(f"[case] batch of {len(views)} views", flush=True)
+    view.set_batch(views)
+    view.render_batch()
+    c = view.counters()
+    got = [view.read_batch_view(i) for i in range(len(views))]
+    total_steps = 0
+    for i, bv in enumerate(views):
+        ref = _plain(view, bv)
+        g = got[i]
+        assert np.array_equal(g.hit_faces, ref.hit_faces), i
+        assert np.array_equal(g.steps, ref.steps), i
+        assert np.array_equal(g.mapping, ref.mapping), i
+        assert np.array_equal(g.ray_casting, ref.ray_casting), i
+        total_steps += int(g.steps.sum())
+    assert c["rays"] == sum(v.width * v.height for v in views)
+    assert c["rk4_steps"] == total_steps
+    # escape mode: uneven path lengths
+    st = np.concatenate([g.steps.ravel() for g in got])
+    assert st.min() < st.max() / 4
+
+
+def test_graph_replays_match_eager(setup):
+    view, views = setup
+    view.set_batch(views)
+    view.render_batch()
+    first = [view.read_batch_view(i) for i in range(len(views))]
+    for rep in range(3):
+        _plain(view, views[rep])          # eager work between replays
+        view.render_batch()
+        for i in range(len(views)):
+            g = view.read_batch_view(i)
+            assert np.array_equal(g.mapping, first[i].mapping), (rep, i)
+            assert np.array_equal(g.hit_faces, first[i].hit_faces), (rep, i)
+
+
+@pytest.mark.parametrize("tuning", [(4, 1, 1, 1), (8, 4, 256, 64), (32, 2, 16, 48)])
+def test_launch_shapes_do_not_change_results(setup, tuning):
+    view, views = setup
+    view.set_batch(views)
+    view.set_batch_tuning(*tuning)
+    view.render_batch()
+    a = [view.read_batch_view(i) for i in range(len(views))]
+    view.set_batch_tuning(16, 1, 32, 16)
+    view.render_batch()
+    for i in range(len(views)):
+        b = view.read_batch_view(i)
+        assert np.array_equal(a[i].mapping, b.mapping) and np.array_equal(a[i].hit_faces, b.hit_faces)
+
+
+def test_autotuner_picks_a_valid_shape(setup):
+    view, views = setup
+    view.set_batch(views)
+    t, ms = view.autotune_batch(downscale=2)
+    assert ms > 0
+    assert t.waves_per_block in (1, 2, 4) and 1 <= t.refill_min <= 64 and t.compact_every >= 1
+    assert view.batch_tuning().as_dict() == t.as_dict()
+    view.render_batch()
+    view.read_batch_view(0)
+
+
+def test_batch_argument_errors(setup):
+    view, views = setup
+    bad = configs.batch_views(num_views=1, seed=0, scale=16)
+    bad[0].field = 0
+    with pytest.raises(L.LinonError):
+        view.set_batch(bad)
+    with pytest.raises(L.LinonError):
+        view.set_batch_tuning(0, 1, 1, 1)

@@ -31,6 +31,7 @@ def _case(view, W, H, cam, st, ig, field, verts, faces, label, accel=L.ACCEL_AUT
     assert info["low_lambda"] > 0, info
     assert info["hit_ok"], info
     assert info["exit_ok"] >= HIT_MIN_AGREE, info
+    assert info["q99_ok"], info
     assert info["color_maxdiff"] <= 1, info
     # rays whose adaptive steps were chosen by rounding noise still find the same faces
     assert info["noisy_hit_agree"] >= 0.95, info
@@ -162,7 +163,7 @@ def test_high_accuracy_render(view):
     ref, ref64 = run_oracles(cam, L.Settings.make(),
                              L.Integrator.make(h_initial=0.001, samples_per_ray=0, sample_steps=0), 3, v, f, W, H)
     info = compare(gpu, ref, ref64, "enhance")
-    assert info["gated"] > 0 and info["hit_ok"] and info["exit_ok"] >= HIT_MIN_AGREE
+    assert info["gated"] > 0 and info["hit_ok"] and info["exit_ok"] >= HIT_MIN_AGREE and info["q99_ok"]
     assert int(np.median(gpu.steps)) >= 5000
 
 
@@ -204,6 +205,7 @@ def test_config2_full_resolution_sample(field):
                O.as_oracle_struct(wl.integrator(), O.IntegratorIn))
     r32 = O.render_pixels(c, s, i, L.FIELDS[field], v, f, wl.width, wl.height, px, 16)
     m64 = O.render_pixels(c, s, i, L.FIELDS[field], v, f, wl.width, wl.height, px, 16, precision="f64")["mapping"]
+    mf = O.render_pixels(c, s, i, L.FIELDS[field], v, f, wl.width, wl.height, px, 16, precision="f32fma")["mapping"]
     class G:
         pass
     g = G()
@@ -212,12 +214,13 @@ def test_config2_full_resolution_sample(field):
     g.ray_casting = fr.ray_casting[ty, gx][None]
     ref = {"mapping": r32["mapping"][None], "hit": r32["hit"][None], "noise_flag": np.zeros((1, len(px)), np.uint8),
            "color8": g.ray_casting}
-    ref64 = {"mapping": m64[None]}
+    ref64 = {"mapping": m64[None], "mapping_f32fma": mf[None]}
     lam = fr.lyapunov[ty, gx][None]
     info = compare(g, ref, ref64, f"config2-{field}", lam_gpu=lam)
     assert info["gated"] > 100, info
     assert info["hit_ok"], info
     assert info["exit_ok"] >= HIT_MIN_AGREE, info
+    assert info["q99_ok"], info
 
 
 @pytest.mark.parametrize("field", ["swirl", "lens", "hot_air", "warp"])
