@@ -135,6 +135,37 @@ typedef struct linon_device_buffers {
     uint32_t width, height;
 } linon_device_buffers;
 
+/* Learned flow-map surrogate of one field (no counterpart in the reference).  A Fourier-feature
+ * MLP (16 frequencies of the 6-float entry state = 198 features, 4 x 256 SiLU layers, 7 outputs:
+ * exit point, exit direction, confidence) maps a ray's entry into the sphere of `radius` to its
+ * exit, trained on uniform entry states labelled by the FP32 RK4 kernel (escape mode, no mesh,
+ * step h).  Hidden layers run on BF16 MFMA with FP32 accumulation; the output layer is FP32. */
+typedef struct linon_surrogate_config {
+    int32_t field;             /* predefined field id */
+    int32_t train_rays;        /* training-set size (hold-out: 1%, at most 65536) */
+    int32_t batch;             /* mini-batch rows, a multiple of 64 */
+    int32_t steps;             /* Adam steps */
+    float lr;                  /* Adam learning rate */
+    float h;                   /* RK4 step of the labels and of the frame's RK4 legs */
+    int32_t max_steps;         /* RK4 iteration cap per leg */
+    float radius;              /* bounding sphere */
+    float conf_threshold;      /* frame: rays below it go to RK4 */
+    float lyapunov_threshold;  /* frame: lambda (overlay's exponent) above it -> RK4 */
+    float tolerance;           /* exit-point error the confidence head certifies */
+    float rejudge_fraction;    /* frame: share of accepted rays re-run with RK4 to measure error */
+    uint32_t seed;
+    int32_t pad[3];
+} linon_surrogate_config;
+
+typedef struct linon_surrogate_stats {
+    uint64_t train_rays;
+    float train_loss, label_seconds, train_seconds;
+    float holdout_err_q50, holdout_err_q99, holdout_frac_within_tol;
+    uint64_t frame_rays, entered, accepted, conf_fallback, lyapunov_fallback, rejudged;
+    float rejudge_err_q50, rejudge_err_q99, rejudge_err_max, rejudge_frac_within_tol;
+    float frame_ms, approach_ms, mlp_ms, fallback_ms, overlay_ms, rejudge_ms;
+} linon_surrogate_stats;
+
 typedef struct linon_context linon_context;
 
 /* One view of a batch: a camera, a resolution, a predefined field and a step size.  The
@@ -217,6 +248,24 @@ linon_status linon_get_batch_tuning(const linon_context* ctx, linon_batch_tuning
  * CU, waves per block, steps per compaction round and refill threshold) and re-captures the
  * batch with it.  best_ms: the proxy batch's time with the chosen shape. */
 linon_status linon_autotune_batch(linon_context* ctx, int downscale, linon_batch_tuning* best, float* best_ms);
+
+/* Surrogate (no counterpart in the reference).  linon_surrogate_train generates the data set,
+ * trains, and reports hold-out errors in `stats`.  linon_render_surrogate renders the current
+ * view without the mesh (mapping texture + Lyapunov overlay; the ray-cast colour is black):
+ * RK4 to the sphere, MLP for the rays that entered, RK4 for unconfident and high-lambda rays,
+ * and RK4 on a sample of the accepted rays to measure the surrogate's error.  The comparable
+ * plain frame is linon_render with an escape-mode integrator (need_mapping) and an empty mesh. */
+linon_status linon_surrogate_train(linon_context* ctx, const linon_surrogate_config* cfg,
+                                   linon_surrogate_stats* stats);
+linon_status linon_render_surrogate(linon_context* ctx, void* stream);
+linon_status linon_get_surrogate_stats(linon_context* ctx, linon_surrogate_stats* out);
+
+/* Diagnostics: the surrogate's BF16-MFMA GEMM on host buffers (bf16 bit patterns in, fp32
+ * out): C = A * B (+ bias; with silu != 0: C = z = A*B + bias and C_bf16 = bf16(silu(z))).
+ * A is [M][K] (a_trans = 0) or [K][M]; B is [K][N] (b_trans = 0) or [N][K]; the K range may
+ * be split in `splits` chunks (multiples of 32) that are reduced afterwards. */
+linon_status linon_selftest_gemm(int device, const uint16_t* A, const uint16_t* B, int M, int N, int K, int a_trans,
+                                 int b_trans, int splits, const float* bias, int silu, float* C, uint16_t* C_bf16);
 
 /* Read-back (the reference keeps these on the GPU; sizes are checked against W*H). */
 linon_status linon_read_final(linon_context* ctx, uint8_t* rgba, size_t bytes);

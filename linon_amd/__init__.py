@@ -174,8 +174,42 @@ class BatchTuning(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class SurrogateConfig(C.Structure):
+    """linon_surrogate_config: the learned flow-map surrogate of one field."""
+    _fields_ = [("field", C.c_int32), ("train_rays", C.c_int32), ("batch", C.c_int32), ("steps", C.c_int32),
+                ("lr", C.c_float), ("h", C.c_float), ("max_steps", C.c_int32), ("radius", C.c_float),
+                ("conf_threshold", C.c_float), ("lyapunov_threshold", C.c_float), ("tolerance", C.c_float),
+                ("rejudge_fraction", C.c_float), ("seed", C.c_uint32), ("pad", C.c_int32 * 3)]
+
+    @classmethod
+    def make(cls, field, train_rays=1 << 24, batch=65536, steps=2000, lr=1e-3, h=0.005, max_steps=4000,
+             radius=2.0, conf_threshold=0.9, lyapunov_threshold=0.0955, tolerance=1e-3, rejudge_fraction=1 / 64,
+             seed=0):
+        c = cls()
+        c.field = FIELDS[field] if isinstance(field, str) else int(field)
+        c.train_rays, c.batch, c.steps, c.lr, c.h = train_rays, batch, steps, lr, h
+        c.max_steps, c.radius, c.conf_threshold = max_steps, radius, conf_threshold
+        c.lyapunov_threshold, c.tolerance, c.rejudge_fraction, c.seed = lyapunov_threshold, tolerance, \
+            rejudge_fraction, seed
+        return c
+
+
+class SurrogateStats(C.Structure):
+    _fields_ = [("train_rays", C.c_uint64), ("train_loss", C.c_float), ("label_seconds", C.c_float),
+                ("train_seconds", C.c_float), ("holdout_err_q50", C.c_float), ("holdout_err_q99", C.c_float),
+                ("holdout_frac_within_tol", C.c_float), ("frame_rays", C.c_uint64), ("entered", C.c_uint64),
+                ("accepted", C.c_uint64), ("conf_fallback", C.c_uint64), ("lyapunov_fallback", C.c_uint64),
+                ("rejudged", C.c_uint64), ("rejudge_err_q50", C.c_float), ("rejudge_err_q99", C.c_float),
+                ("rejudge_err_max", C.c_float), ("rejudge_frac_within_tol", C.c_float), ("frame_ms", C.c_float),
+                ("approach_ms", C.c_float), ("mlp_ms", C.c_float), ("fallback_ms", C.c_float),
+                ("overlay_ms", C.c_float), ("rejudge_ms", C.c_float)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 assert C.sizeof(Camera) == 112 and C.sizeof(Settings) == 24 and C.sizeof(Integrator) == 48
-assert C.sizeof(View) == 144 and C.sizeof(BatchTuning) == 16
+assert C.sizeof(View) == 144 and C.sizeof(BatchTuning) == 16 and C.sizeof(SurrogateConfig) == 64
 
 _lib = None
 
@@ -218,6 +252,11 @@ SIGNATURES = {
     "linon_set_batch_tuning": (C.c_int, [_P, C.POINTER(BatchTuning)]),
     "linon_get_batch_tuning": (C.c_int, [_P, C.POINTER(BatchTuning)]),
     "linon_autotune_batch": (C.c_int, [_P, C.c_int, C.POINTER(BatchTuning), C.POINTER(C.c_float)]),
+    "linon_surrogate_train": (C.c_int, [_P, C.POINTER(SurrogateConfig), C.POINTER(SurrogateStats)]),
+    "linon_render_surrogate": (C.c_int, [_P, _P]),
+    "linon_get_surrogate_stats": (C.c_int, [_P, C.POINTER(SurrogateStats)]),
+    "linon_selftest_gemm": (C.c_int, [C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P,
+                                      C.c_int, _P, _P]),
     "linon_reset_timings": (C.c_int, [_P]),
 }
 
@@ -435,6 +474,20 @@ class MainView:
         ms = C.c_float(0)
         self._check(self._lib.linon_autotune_batch(self._h, downscale, C.byref(t), C.byref(ms)))
         return t, ms.value
+
+    # -- learned flow-map surrogate ---------------------------------------------------------
+    def train_surrogate(self, cfg: SurrogateConfig) -> dict:
+        st = SurrogateStats()
+        self._check(self._lib.linon_surrogate_train(self._h, C.byref(cfg), C.byref(st)))
+        return st.as_dict()
+
+    def render_surrogate(self, stream: int | None = None) -> None:
+        self._check(self._lib.linon_render_surrogate(self._h, C.c_void_p(stream)))
+
+    def surrogate_stats(self) -> dict:
+        st = SurrogateStats()
+        self._check(self._lib.linon_get_surrogate_stats(self._h, C.byref(st)))
+        return st.as_dict()
 
     def timings(self):
         """(raycast_ms, render_ms) arrays of the renders since reset_timings()."""

@@ -34,6 +34,50 @@ struct BatchState {
     void release();
 };
 
+// Learned flow-map surrogate of one field: parameters, optimiser state, training and frame
+// buffers (all device memory, owned by the context).
+struct Surrogate {
+    linon_surrogate_config cfg{};
+    bool trained = false;
+    int adam_t = 0;
+    unsigned frames = 0;
+    float* W[5] = {};  // FP32 master weights [K][N]; layer 4 is the output layer
+    float* b[5] = {};
+    float* mW[5] = {};
+    float* vW[5] = {};
+    float* mb[5] = {};
+    float* vb[5] = {};
+    float* gW[5] = {};
+    float* gb[5] = {};
+    unsigned short* Wt[4] = {};  // BF16 copies [N][K] of the hidden layers
+    float* d_freqs = nullptr;
+    // training
+    float *d_states = nullptr, *d_targets = nullptr, *d_bstates = nullptr, *d_btargets = nullptr;
+    unsigned short* d_feat = nullptr;
+    unsigned short* d_act[5] = {};
+    float* d_z[4] = {};
+    float *d_out = nullptr, *d_dout = nullptr, *d_err = nullptr, *d_part = nullptr, *d_loss = nullptr;
+    unsigned short* d_dz[2] = {};
+    // frame
+    int frame_px = 0;
+    float* f_entry = nullptr;
+    int* f_entered = nullptr;
+    float* f_states = nullptr;
+    int* f_texel = nullptr;
+    int* f_count = nullptr;
+    float* f_pred = nullptr;
+    int* f_accepted = nullptr;
+    int* f_list = nullptr;
+    float* f_exits = nullptr;
+    float* f_errs = nullptr;
+    unsigned short *f_feat = nullptr, *f_act0 = nullptr, *f_act1 = nullptr;
+    float* f_out = nullptr;
+    int* f_steps = nullptr;
+    linon_surrogate_stats train_stats{};
+    linon_surrogate_stats frame_stats{};
+    void release();
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -99,6 +143,16 @@ struct Context {
     linon_status batch_prepare(BatchState& B, const linon_view* views, int n);
     linon_status batch_launch(BatchState& B, const linon_batch_tuning& T, hipStream_t s);
     linon_status batch_capture();
+
+    Surrogate sg;
+    linon_status surrogate_alloc_params();
+    linon_status surrogate_init_params(unsigned long long seed);
+    linon_status surrogate_forward(int n, const unsigned short* feat, unsigned short* const act[5],
+                                   float* const z[4], hipStream_t s);
+    linon_status surrogate_train(const linon_surrogate_config& cfg, linon_surrogate_stats* stats);
+    linon_status surrogate_eval(const float* states, const float* targets, int n, std::vector<float>& err);
+    linon_status surrogate_frame(hipStream_t s);
+    StepParams surrogate_step_params() const;
 
     void fill_scene(SceneParams& S) const;
     void fill_raycast_params(RaycastParams& P, const linon_integrator& ig) const;

@@ -1,0 +1,651 @@
+// Learned flow-map surrogate: host side (training loop, frame pipeline, statistics, save/load).
+//
+// Training follows the usual data-parallel recipe on one device: uniform entry states ->
+// labels from the FP32 RK4 flow-map kernel -> mini-batches gathered by a counter-based hash ->
+// forward (four BF16-MFMA GEMMs with fused bias+SiLU, FP32 output layer) -> loss (MSE of the
+// scaled exit state + logistic loss of the confidence head against "error < tolerance") ->
+// backward (FP32 output layer, BF16-MFMA GEMMs for dX with the SiLU' gate fused, split-K GEMMs
+// for dW) -> Adam on FP32 master weights, which also refreshes the BF16 [N][K] weight copies.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/linon.h"
+#include "context.hpp"
+#include "kernels.hpp"
+#include "surrogate_kernels.hpp"
+
+namespace linon {
+
+namespace {
+
+#define LS_CHECK(ctx, call)                                                                      \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return (ctx)->fail(LINON_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+const int kLayerK[4] = {LS_FEATS_PAD, LS_HIDDEN, LS_HIDDEN, LS_HIDDEN};
+
+template <class T>
+hipError_t dalloc(T*& p, size_t n) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    return hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+float host_bf16_round(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+
+}  // namespace
+
+void Surrogate::release() {
+    for (int l = 0; l < 5; ++l)
+        for (float** p : {&W[l], &b[l], &mW[l], &vW[l], &mb[l], &vb[l], &gW[l], &gb[l]}) {
+            if (*p) (void)hipFree(*p);
+            *p = nullptr;
+        }
+    for (int l = 0; l < 4; ++l) {
+        if (Wt[l]) (void)hipFree(Wt[l]);
+        Wt[l] = nullptr;
+    }
+    void* bufs[] = {d_freqs, d_states, d_targets, d_bstates, d_btargets, d_feat, d_act[0], d_act[1], d_act[2],
+                    d_act[3], d_act[4], d_z[0], d_z[1], d_z[2], d_z[3], d_out, d_dout, d_err, d_dz[0], d_dz[1],
+                    d_part, d_loss, f_entry, f_entered, f_states, f_texel, f_count, f_pred, f_accepted, f_list,
+                    f_exits, f_errs, f_feat, f_act0, f_act1, f_out, f_steps};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    *this = Surrogate();
+}
+
+linon_status Context::surrogate_alloc_params() {
+    Surrogate& S = sg;
+    for (int l = 0; l < 5; ++l) {
+        const int K = l < 4 ? kLayerK[l] : LS_HIDDEN;
+        const int N = l < 4 ? LS_HIDDEN : LS_OUT;
+        LS_CHECK(this, dalloc(S.W[l], (size_t)K * N));
+        LS_CHECK(this, dalloc(S.b[l], N));
+        LS_CHECK(this, dalloc(S.mW[l], (size_t)K * N));
+        LS_CHECK(this, dalloc(S.vW[l], (size_t)K * N));
+        LS_CHECK(this, dalloc(S.mb[l], N));
+        LS_CHECK(this, dalloc(S.vb[l], N));
+        LS_CHECK(this, dalloc(S.gW[l], (size_t)K * N));
+        LS_CHECK(this, dalloc(S.gb[l], N));
+        LS_CHECK(this, hipMemset(S.mW[l], 0, (size_t)K * N * 4));
+        LS_CHECK(this, hipMemset(S.vW[l], 0, (size_t)K * N * 4));
+        LS_CHECK(this, hipMemset(S.mb[l], 0, N * 4));
+        LS_CHECK(this, hipMemset(S.vb[l], 0, N * 4));
+        if (l < 4) LS_CHECK(this, dalloc(S.Wt[l], (size_t)K * N));
+    }
+    LS_CHECK(this, dalloc(S.d_freqs, LS_FREQS));
+    std::vector<float> fr(LS_FREQS);
+    // geometric frequencies from pi/2 to 16 pi over the unit-scaled coordinates
+    for (int f = 0; f < LS_FREQS; ++f) fr[f] = (float)(1.5707963267948966 * std::pow(32.0, f / (double)(LS_FREQS - 1)));
+    LS_CHECK(this, hipMemcpy(S.d_freqs, fr.data(), LS_FREQS * 4, hipMemcpyHostToDevice));
+    return LINON_OK;
+}
+
+// Kaiming-uniform initialisation (SplitMix64 stream from the seed) of the master weights and
+// the BF16 [N][K] copies; zero biases.
+linon_status Context::surrogate_init_params(unsigned long long seed) {
+    Surrogate& S = sg;
+    unsigned long long st = seed;
+    auto next = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        unsigned long long z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return (double)((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    for (int l = 0; l < 5; ++l) {
+        const int K = l < 4 ? kLayerK[l] : LS_HIDDEN;
+        const int N = l < 4 ? LS_HIDDEN : LS_OUT;
+        const int fan_in = l == 0 ? LS_FEATS : K;
+        const double bound = (l < 4 ? std::sqrt(6.0 / fan_in) : std::sqrt(1.0 / fan_in));
+        std::vector<float> w((size_t)K * N), wt((size_t)K * N, 0.0f);
+        std::vector<uint16_t> wtb((size_t)K * N, 0);
+        for (int k = 0; k < K; ++k)
+            for (int n = 0; n < N; ++n) {
+                float v = (l == 0 && k >= LS_FEATS) ? 0.0f : (float)((2.0 * next() - 1.0) * bound);
+                w[(size_t)k * N + n] = v;
+                if (l < 4) {
+                    float r = host_bf16_round(v);
+                    uint32_t u;
+                    std::memcpy(&u, &r, 4);
+                    wtb[(size_t)n * K + k] = (uint16_t)(u >> 16);
+                }
+            }
+        LS_CHECK(this, hipMemcpy(S.W[l], w.data(), w.size() * 4, hipMemcpyHostToDevice));
+        LS_CHECK(this, hipMemset(S.b[l], 0, N * 4));
+        if (l < 4) LS_CHECK(this, hipMemcpy(S.Wt[l], wtb.data(), wtb.size() * 2, hipMemcpyHostToDevice));
+    }
+    S.adam_t = 0;
+    return LINON_OK;
+}
+
+// Forward of the hidden layers for n rows of features: act[0] = features, act[l+1] = silu(z).
+// With z != null the pre-activations are kept (training).
+linon_status Context::surrogate_forward(int n, const lm::u16* feat, lm::u16* const act[5], float* const z[4],
+                                        hipStream_t s) {
+    Surrogate& S = sg;
+    const lm::u16* x = feat;
+    for (int l = 0; l < 4; ++l) {
+        lm::GemmParams P;
+        std::memset(&P, 0, sizeof(P));
+        P.A = x;
+        P.B = S.Wt[l];
+        P.M = n;
+        P.N = LS_HIDDEN;
+        P.K = kLayerK[l];
+        P.lda = kLayerK[l];
+        P.ldb = kLayerK[l];
+        P.a_trans = 0;
+        P.b_trans = 1;
+        P.k_per_split = kLayerK[l];
+        P.epi = LM_EPI_BIAS_SILU;
+        P.bias = S.b[l];
+        P.out_bf16 = act[l + 1];
+        P.z_out = z ? z[l] : nullptr;
+        P.ldc = LS_HIDDEN;
+        LS_CHECK(this, sg_gemm(P, 1, s));
+        x = act[l + 1];
+    }
+    return LINON_OK;
+}
+
+linon_status Context::surrogate_train(const linon_surrogate_config& cfg, linon_surrogate_stats* stats) {
+    if (!sg_field_supported(cfg.field))
+        return fail(LINON_ERR_ARGUMENT, "surrogate: field must be swirl, lens, hot-air, warp, identity or "
+                                        "mirage (spherical sigmoid)");
+    if (cfg.train_rays < 1024 || cfg.batch < 256 || cfg.batch > (1 << 20) || cfg.batch % 64 || cfg.steps < 0 ||
+        !(cfg.lr > 0.0f) || !(cfg.h > 0.0f) || cfg.max_steps < 1 || !(cfg.radius > 0.0f))
+        return fail(LINON_ERR_ARGUMENT, "surrogate: bad configuration");
+    Surrogate& S = sg;
+    S.release();
+    S.cfg = cfg;
+    const int N = cfg.train_rays, B = cfg.batch;
+    const int n_hold = std::min(65536, std::max(1024, N / 100));
+    const int n_train = N - n_hold;
+    hipStream_t s = stream;
+    LS_CHECK(this, hipSetDevice(device));
+    linon_status st = surrogate_alloc_params();
+    if (st != LINON_OK) return st;
+    st = surrogate_init_params(cfg.seed * 7919ull + 17ull);
+    if (st != LINON_OK) return st;
+
+    // ---- data: uniform entry states, labels from the FP32 RK4 flow map ----
+    auto t0 = std::chrono::steady_clock::now();
+    LS_CHECK(this, dalloc(S.d_states, (size_t)N * 6));
+    LS_CHECK(this, dalloc(S.d_targets, (size_t)N * 6));
+    LS_CHECK(this, sg_sample_states(cfg.seed, N, cfg.radius, cfg.h, S.d_states, s));
+    ls::FlowmapParams F;
+    std::memset(&F, 0, sizeof(F));
+    F.step = surrogate_step_params();
+    F.states = S.d_states;
+    F.n = N;
+    F.exits = S.d_targets;
+    LS_CHECK(this, sg_flowmap(cfg.field, F, s));
+    LS_CHECK(this, hipStreamSynchronize(s));
+    auto t1 = std::chrono::steady_clock::now();
+
+    // ---- buffers ----
+    LS_CHECK(this, dalloc(S.d_bstates, (size_t)B * 6));
+    LS_CHECK(this, dalloc(S.d_btargets, (size_t)B * 6));
+    LS_CHECK(this, dalloc(S.d_feat, (size_t)B * LS_FEATS_PAD));
+    for (int l = 1; l < 5; ++l) LS_CHECK(this, dalloc(S.d_act[l], (size_t)B * LS_HIDDEN));
+    for (int l = 0; l < 4; ++l) LS_CHECK(this, dalloc(S.d_z[l], (size_t)B * LS_HIDDEN));
+    LS_CHECK(this, dalloc(S.d_out, (size_t)B * LS_OUT));
+    LS_CHECK(this, dalloc(S.d_dout, (size_t)B * LS_OUT));
+    LS_CHECK(this, dalloc(S.d_err, (size_t)B));
+    LS_CHECK(this, dalloc(S.d_dz[0], (size_t)B * LS_HIDDEN));
+    LS_CHECK(this, dalloc(S.d_dz[1], (size_t)B * LS_HIDDEN));
+    const int ksplit = 1024;  // batch rows per split-K chunk of a weight gradient
+    const int splits = (B + ksplit - 1) / ksplit;
+    LS_CHECK(this, dalloc(S.d_part, (size_t)splits * LS_FEATS_PAD * LS_HIDDEN));
+    LS_CHECK(this, dalloc(S.d_loss, 1));
+    S.d_act[0] = S.d_feat;
+
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float conf_w = 0.1f;
+    float last_loss = 0.0f;
+    for (int step = 0; step < cfg.steps; ++step) {
+        LS_CHECK(this, sg_gather(S.d_states, S.d_targets, n_train, cfg.seed ^ 0x5DEECE66Dull, step, B, cfg.radius,
+                                 S.d_bstates, S.d_btargets, s));
+        ls::SurrogateIO io{S.d_bstates, B, cfg.radius, S.d_freqs, S.d_feat};
+        LS_CHECK(this, sg_features(io, s));
+        lm::u16* const acts[5] = {S.d_feat, S.d_act[1], S.d_act[2], S.d_act[3], S.d_act[4]};
+        float* const zs[4] = {S.d_z[0], S.d_z[1], S.d_z[2], S.d_z[3]};
+        st = surrogate_forward(B, S.d_feat, acts, zs, s);
+        if (st != LINON_OK) return st;
+        LS_CHECK(this, sg_out_layer(S.d_act[4], S.W[4], S.b[4], B, S.d_out, s));
+        const bool want_loss = (step % 50 == 0) || step == cfg.steps - 1;
+        if (want_loss) LS_CHECK(this, hipMemsetAsync(S.d_loss, 0, 4, s));
+        LS_CHECK(this, sg_loss(S.d_out, S.d_btargets, B, cfg.radius, cfg.tolerance, conf_w, S.d_dout, S.d_err,
+                               S.d_loss, s));
+        // output layer (FP32)
+        LS_CHECK(this, sg_out_layer_grad(S.d_act[4], S.d_dout, B, S.gW[4], S.gb[4], s));
+        LS_CHECK(this, sg_out_layer_back(S.d_dout, S.W[4], S.d_z[3], B, S.d_dz[0], s));
+        // hidden layers, top down; dz[cur] holds dZ of layer l
+        int cur = 0;
+        for (int l = 3; l >= 0; --l) {
+            const int K = kLayerK[l];
+            lm::GemmParams G;  // dW = X^T dZ, split over the batch
+            std::memset(&G, 0, sizeof(G));
+            G.A = acts[l];
+            G.B = S.d_dz[cur];
+            G.M = K;
+            G.N = LS_HIDDEN;
+            G.K = B;
+            G.lda = K;
+            G.ldb = LS_HIDDEN;
+            G.a_trans = 1;
+            G.b_trans = 0;
+            G.k_per_split = ksplit;
+            G.epi = LM_EPI_PARTIAL;
+            G.out_f32 = S.d_part;
+            G.ldc = LS_HIDDEN;
+            LS_CHECK(this, sg_gemm(G, splits, s));
+            LS_CHECK(this, sg_reduce(S.d_part, splits, K * LS_HIDDEN, S.gW[l], s));
+            LS_CHECK(this, sg_colsum(S.d_dz[cur], B, LS_HIDDEN, S.gb[l], s));
+            if (l > 0) {  // dZ of layer l-1 = (dZ W^T) * silu'(z_{l-1})
+                lm::GemmParams D;
+                std::memset(&D, 0, sizeof(D));
+                D.A = S.d_dz[cur];
+                D.B = S.Wt[l];
+                D.M = B;
+                D.N = K;
+                D.K = LS_HIDDEN;
+                D.lda = LS_HIDDEN;
+                D.ldb = K;
+                D.a_trans = 0;
+                D.b_trans = 0;
+                D.k_per_split = LS_HIDDEN;
+                D.epi = LM_EPI_SILU_GRAD;
+                D.z_in = S.d_z[l - 1];
+                D.out_bf16 = S.d_dz[cur ^ 1];
+                D.ldc = K;
+                LS_CHECK(this, sg_gemm(D, 1, s));
+                cur ^= 1;
+            }
+        }
+        // Adam
+        ++S.adam_t;
+        const float bc1 = 1.0f - std::pow(b1, (float)S.adam_t), bc2 = 1.0f - std::pow(b2, (float)S.adam_t);
+        for (int l = 0; l < 5; ++l) {
+            const int K = l < 4 ? kLayerK[l] : LS_HIDDEN;
+            const int Nn = l < 4 ? LS_HIDDEN : LS_OUT;
+            LS_CHECK(this, sg_adam(S.W[l], S.mW[l], S.vW[l], S.gW[l], K, Nn, cfg.lr, b1, b2, eps, bc1, bc2,
+                                   l < 4 ? S.Wt[l] : nullptr, K, s));
+            LS_CHECK(this, sg_adam(S.b[l], S.mb[l], S.vb[l], S.gb[l], 1, Nn, cfg.lr, b1, b2, eps, bc1, bc2, nullptr,
+                                   0, s));
+        }
+        if (want_loss) {
+            LS_CHECK(this, hipMemcpyAsync(&last_loss, S.d_loss, 4, hipMemcpyDeviceToHost, s));
+            LS_CHECK(this, hipStreamSynchronize(s));
+            if (!std::isfinite(last_loss)) return fail(LINON_ERR_STATE, "surrogate: training diverged (loss NaN)");
+        }
+    }
+    LS_CHECK(this, hipStreamSynchronize(s));
+    auto t2 = std::chrono::steady_clock::now();
+
+    // ---- hold-out error (the last n_hold states, never sampled for training) ----
+    std::vector<float> err;
+    st = surrogate_eval(S.d_states + 6 * (size_t)n_train, S.d_targets + 6 * (size_t)n_train, n_hold, err);
+    if (st != LINON_OK) return st;
+    S.trained = true;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->train_loss = last_loss;
+        stats->label_seconds = std::chrono::duration<float>(t1 - t0).count();
+        stats->train_seconds = std::chrono::duration<float>(t2 - t1).count();
+        std::vector<float> e = err;
+        std::sort(e.begin(), e.end());
+        stats->holdout_err_q50 = e[e.size() / 2];
+        stats->holdout_err_q99 = e[std::min(e.size() - 1, (size_t)(0.99 * e.size()))];
+        stats->holdout_frac_within_tol =
+            (float)(std::lower_bound(e.begin(), e.end(), cfg.tolerance) - e.begin()) / (float)e.size();
+        stats->train_rays = (uint64_t)N;
+    }
+    S.train_stats = stats ? *stats : linon_surrogate_stats{};
+    return LINON_OK;
+}
+
+StepParams Context::surrogate_step_params() const {
+    StepParams I;
+    std::memset(&I, 0, sizeof(I));
+    I.h_initial = sg.cfg.h;
+    I.t_max = 3.0e38f;
+    I.field_weight = 1.0f;
+    I.escape_r2 = sg.cfg.radius * sg.cfg.radius;
+    I.adaptive = 0;
+    I.max_steps = sg.cfg.max_steps;
+    I.mode = LINON_MODE_ESCAPE;
+    I.need_mapping = 1;
+    return I;
+}
+
+// Exit-point error of the surrogate on n states against their labels (host copy of errors).
+linon_status Context::surrogate_eval(const float* states, const float* targets, int n, std::vector<float>& err) {
+    Surrogate& S = sg;
+    hipStream_t s = stream;
+    lm::u16* feat = nullptr;
+    lm::u16 *a0 = nullptr, *a1 = nullptr;
+    float *out = nullptr, *dout = nullptr, *e = nullptr, *loss = nullptr, *scaled = nullptr;
+    auto cleanup = [&]() {
+        for (void* p : {(void*)feat, (void*)a0, (void*)a1, (void*)out, (void*)dout, (void*)e, (void*)loss,
+                        (void*)scaled})
+            if (p) (void)hipFree(p);
+    };
+    if (dalloc(feat, (size_t)n * LS_FEATS_PAD) != hipSuccess || dalloc(a0, (size_t)n * LS_HIDDEN) != hipSuccess ||
+        dalloc(a1, (size_t)n * LS_HIDDEN) != hipSuccess || dalloc(out, (size_t)n * LS_OUT) != hipSuccess ||
+        dalloc(dout, (size_t)n * LS_OUT) != hipSuccess || dalloc(e, (size_t)n) != hipSuccess ||
+        dalloc(loss, 1) != hipSuccess || dalloc(scaled, (size_t)n * 6) != hipSuccess) {
+        cleanup();
+        return fail(LINON_ERR_HIP, "surrogate_eval: out of device memory");
+    }
+    linon_status st = LINON_OK;
+    do {
+        ls::SurrogateIO io{states, n, S.cfg.radius, S.d_freqs, feat};
+        if (sg_features(io, s) != hipSuccess) { st = fail(LINON_ERR_HIP, "eval features"); break; }
+        lm::u16* const acts[5] = {feat, a0, a1, a0, a1};
+        st = surrogate_forward(n, feat, acts, nullptr, s);
+        if (st != LINON_OK) break;
+        if (sg_out_layer(a1, S.W[4], S.b[4], n, out, s) != hipSuccess) { st = fail(LINON_ERR_HIP, "eval out"); break; }
+        // labels scaled like the training targets
+        std::vector<float> t((size_t)n * 6);
+        if (hipMemcpy(t.data(), targets, t.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            st = fail(LINON_ERR_HIP, "eval copy");
+            break;
+        }
+        for (size_t i = 0; i < (size_t)n; ++i)
+            for (int k = 0; k < 3; ++k) t[6 * i + k] /= S.cfg.radius;
+        if (hipMemcpy(scaled, t.data(), t.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(loss, 0, 4) != hipSuccess) {
+            st = fail(LINON_ERR_HIP, "eval copy");
+            break;
+        }
+        if (sg_loss(out, scaled, n, S.cfg.radius, S.cfg.tolerance, 0.1f, dout, e, loss, s) != hipSuccess) {
+            st = fail(LINON_ERR_HIP, "eval loss");
+            break;
+        }
+        err.resize(n);
+        if (hipMemcpy(err.data(), e, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            st = fail(LINON_ERR_HIP, "eval copy");
+    } while (false);
+    (void)hipDeviceSynchronize();
+    cleanup();
+    return st;
+}
+
+// One surrogate frame of the current view (no mesh: the mapping texture and the overlay):
+//   1. RK4 from the camera to the bounding sphere (rays that never enter end there);
+//   2. the MLP on the rays that entered (compacted), accepting confident predictions;
+//   3. RK4 from the entry state for rays below the confidence threshold;
+//   4. the overlay's Lyapunov exponent on that mapping; accepted rays above the threshold go
+//      to RK4 as well, and the overlay runs again;
+//   5. re-judging: a hash-spread sample of the accepted rays is also run with RK4 and the
+//      surrogate's exit-point error on it recorded.
+linon_status Context::surrogate_frame(hipStream_t s) {
+    Surrogate& S = sg;
+    if (!S.trained) return fail(LINON_ERR_STATE, "render_surrogate: no trained surrogate (linon_surrogate_train)");
+    const int W = (int)width, H = (int)height, npx = W * H;
+    LS_CHECK(this, hipSetDevice(device));
+    if (S.frame_px != npx) {
+        LS_CHECK(this, dalloc(S.f_entry, (size_t)npx * 6));
+        LS_CHECK(this, dalloc(S.f_entered, (size_t)npx));
+        LS_CHECK(this, dalloc(S.f_states, (size_t)npx * 6));
+        LS_CHECK(this, dalloc(S.f_texel, (size_t)npx));
+        LS_CHECK(this, dalloc(S.f_count, 4));
+        LS_CHECK(this, dalloc(S.f_pred, (size_t)npx * 3));
+        LS_CHECK(this, dalloc(S.f_accepted, (size_t)npx));
+        LS_CHECK(this, dalloc(S.f_list, (size_t)npx));
+        LS_CHECK(this, dalloc(S.f_exits, (size_t)npx * 6));
+        LS_CHECK(this, dalloc(S.f_errs, (size_t)npx));
+        LS_CHECK(this, dalloc(S.f_feat, (size_t)npx * LS_FEATS_PAD));
+        LS_CHECK(this, dalloc(S.f_act0, (size_t)npx * LS_HIDDEN));
+        LS_CHECK(this, dalloc(S.f_act1, (size_t)npx * LS_HIDDEN));
+        LS_CHECK(this, dalloc(S.f_out, (size_t)npx * LS_OUT));
+        S.frame_px = npx;
+    }
+    hipEvent_t e[6];
+    for (auto& x : e) LS_CHECK(this, hipEventCreate(&x));
+    auto destroy = [&]() {
+        for (auto& x : e) (void)hipEventDestroy(x);
+    };
+    const StepParams I = surrogate_step_params();
+    const bool q_is_v0 = S.cfg.field == LINON_FIELD_SWIRL || S.cfg.field == LINON_FIELD_WARP ||
+                         (S.cfg.field >= 5 && S.cfg.field <= 7);
+    LS_CHECK(this, hipEventRecord(e[0], s));
+    // 1. approach
+    ls::ApproachParams A;
+    std::memset(&A, 0, sizeof(A));
+    fill_camera(A.cam, camera, width, height);
+    A.step = I;
+    A.radius = S.cfg.radius;
+    A.tiles_x = (W + LINON_TILE - 1) / LINON_TILE;
+    A.entry_states = S.f_entry;
+    A.entered = S.f_entered;
+    A.mapping = d_mapping;
+    A.steps = d_steps;
+    A.q_is_v0 = q_is_v0 ? 1 : 0;
+    LS_CHECK(this, sg_approach(S.cfg.field, A, s));
+    LS_CHECK(this, sg_fill_i32(S.f_count, 1, 0, s));
+    LS_CHECK(this, sg_compact_entered(S.f_entered, S.f_entry, npx, S.f_states, S.f_texel, S.f_count, s));
+    int n_in = 0;
+    LS_CHECK(this, hipMemcpyAsync(&n_in, S.f_count, 4, hipMemcpyDeviceToHost, s));
+    LS_CHECK(this, hipStreamSynchronize(s));
+    LS_CHECK(this, hipEventRecord(e[1], s));
+    // 2. MLP
+    int n_fb = 0;
+    if (n_in > 0) {
+        ls::SurrogateIO io{S.f_states, n_in, S.cfg.radius, S.d_freqs, S.f_feat};
+        LS_CHECK(this, sg_features(io, s));
+        lm::u16* const acts[5] = {S.f_feat, S.f_act0, S.f_act1, S.f_act0, S.f_act1};
+        linon_status st = surrogate_forward(n_in, S.f_feat, acts, nullptr, s);
+        if (st != LINON_OK) {
+            destroy();
+            return st;
+        }
+        LS_CHECK(this, sg_out_layer(S.f_act1, S.W[4], S.b[4], n_in, S.f_out, s));
+        LS_CHECK(this, sg_fill_i32(S.f_count, 1, 0, s));
+        LS_CHECK(this, sg_accept(S.f_out, S.f_texel, n_in, S.cfg.radius, S.cfg.conf_threshold, d_mapping, S.f_pred,
+                                 S.f_list, S.f_count, S.f_accepted, s));
+        LS_CHECK(this, hipMemcpyAsync(&n_fb, S.f_count, 4, hipMemcpyDeviceToHost, s));
+        LS_CHECK(this, hipStreamSynchronize(s));
+    }
+    LS_CHECK(this, hipEventRecord(e[2], s));
+    // 3. RK4 for the unconfident rays
+    ls::FlowmapParams F;
+    std::memset(&F, 0, sizeof(F));
+    F.step = I;
+    F.states = S.f_states;
+    F.exits = S.f_exits;
+    if (n_fb > 0) {
+        F.index = S.f_list;
+        F.n = n_fb;
+        LS_CHECK(this, sg_flowmap(S.cfg.field, F, s));
+        LS_CHECK(this, sg_apply_rk4(S.f_list, n_fb, S.f_exits, S.f_texel, S.f_pred, d_mapping, nullptr, 1, s));
+    }
+    LS_CHECK(this, hipEventRecord(e[3], s));
+    // 4. Lyapunov gate (the overlay computes lambda from the mapping)
+    OverlayParams O;
+    std::memset(&O, 0, sizeof(O));
+    O.width = W;
+    O.height = H;
+    O.overlay_mode = 1;
+    O.delta = settings.central_difference_delta;
+    O.scaling = settings.lyapunov_scaling;
+    O.srgb = srgb_output ? 1 : 0;
+    O.ray_casting = d_color;
+    O.mapping = d_mapping;
+    O.final_color = d_final;
+    O.exponents = d_exponents;
+    O.lyapunov = d_lyap;
+    LS_CHECK(this, sg_fill_i32(reinterpret_cast<int*>(d_color), npx, (int)0xff000000u, s));  // black, alpha 1
+    LS_CHECK(this, launch_overlay(O, s));
+    int n_ly = 0;
+    if (n_in > 0) {
+        LS_CHECK(this, sg_fill_i32(S.f_count, 1, 0, s));
+        LS_CHECK(this, sg_lyapunov_gate(S.f_accepted, S.f_texel, n_in, d_lyap, S.cfg.lyapunov_threshold, S.f_list,
+                                        S.f_count, s));
+        LS_CHECK(this, hipMemcpyAsync(&n_ly, S.f_count, 4, hipMemcpyDeviceToHost, s));
+        LS_CHECK(this, hipStreamSynchronize(s));
+        if (n_ly > 0) {
+            F.index = S.f_list;
+            F.n = n_ly;
+            LS_CHECK(this, sg_flowmap(S.cfg.field, F, s));
+            LS_CHECK(this, sg_apply_rk4(S.f_list, n_ly, S.f_exits, S.f_texel, S.f_pred, d_mapping, nullptr, 1, s));
+            LS_CHECK(this, launch_overlay(O, s));
+        }
+    }
+    LS_CHECK(this, hipEventRecord(e[4], s));
+    // 5. re-judge a sample of the accepted rays
+    int n_rj = 0;
+    std::vector<float> rj;
+    if (n_in > 0 && S.cfg.rejudge_fraction > 0.0f) {
+        const int stride = std::max(1, (int)std::lround(1.0 / S.cfg.rejudge_fraction));
+        LS_CHECK(this, sg_fill_i32(S.f_count, 1, 0, s));
+        LS_CHECK(this, sg_pick_rejudge(S.f_accepted, n_in, stride, 0xC0FFEEull + S.frames, S.f_list, S.f_count, s));
+        LS_CHECK(this, hipMemcpyAsync(&n_rj, S.f_count, 4, hipMemcpyDeviceToHost, s));
+        LS_CHECK(this, hipStreamSynchronize(s));
+        if (n_rj > 0) {
+            F.index = S.f_list;
+            F.n = n_rj;
+            LS_CHECK(this, sg_flowmap(S.cfg.field, F, s));
+            LS_CHECK(this, sg_apply_rk4(S.f_list, n_rj, S.f_exits, S.f_texel, S.f_pred, d_mapping, S.f_errs, 0, s));
+            rj.resize(n_rj);
+            LS_CHECK(this, hipMemcpyAsync(rj.data(), S.f_errs, (size_t)n_rj * 4, hipMemcpyDeviceToHost, s));
+        }
+    }
+    LS_CHECK(this, hipEventRecord(e[5], s));
+    LS_CHECK(this, hipEventSynchronize(e[5]));
+    linon_surrogate_stats& F2 = S.frame_stats;
+    F2 = S.train_stats;
+    F2.frame_rays = (uint64_t)npx;
+    F2.entered = (uint64_t)n_in;
+    F2.conf_fallback = (uint64_t)n_fb;
+    F2.lyapunov_fallback = (uint64_t)n_ly;
+    F2.accepted = (uint64_t)(n_in - n_fb - n_ly);
+    F2.rejudged = (uint64_t)n_rj;
+    float ms[5];
+    for (int k = 0; k < 5; ++k) (void)hipEventElapsedTime(&ms[k], e[k], e[k + 1]);
+    F2.approach_ms = ms[0];
+    F2.mlp_ms = ms[1];
+    F2.fallback_ms = ms[2];
+    F2.overlay_ms = ms[3];
+    F2.rejudge_ms = ms[4];
+    F2.frame_ms = ms[0] + ms[1] + ms[2] + ms[3];
+    if (!rj.empty()) {
+        std::sort(rj.begin(), rj.end());
+        F2.rejudge_err_q50 = rj[rj.size() / 2];
+        F2.rejudge_err_q99 = rj[std::min(rj.size() - 1, (size_t)(0.99 * rj.size()))];
+        F2.rejudge_err_max = rj.back();
+        F2.rejudge_frac_within_tol =
+            (float)(std::lower_bound(rj.begin(), rj.end(), S.cfg.tolerance) - rj.begin()) / (float)rj.size();
+    }
+    ++S.frames;
+    destroy();
+    return LINON_OK;
+}
+
+}  // namespace linon
+
+using linon::Context;
+
+extern "C" {
+
+linon_status linon_surrogate_train(linon_context* ctx, const linon_surrogate_config* cfg,
+                                   linon_surrogate_stats* stats) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return LINON_ERR_ARGUMENT;
+    c->err.clear();
+    if (!cfg) return c->fail(LINON_ERR_ARGUMENT, "surrogate_train: null config");
+    return c->surrogate_train(*cfg, stats);
+}
+
+linon_status linon_render_surrogate(linon_context* ctx, void* stream) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return LINON_ERR_ARGUMENT;
+    c->err.clear();
+    return c->surrogate_frame(stream ? reinterpret_cast<hipStream_t>(stream) : c->stream);
+}
+
+linon_status linon_get_surrogate_stats(linon_context* ctx, linon_surrogate_stats* out) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c || !out) return LINON_ERR_ARGUMENT;
+    *out = c->sg.frames ? c->sg.frame_stats : c->sg.train_stats;
+    return LINON_OK;
+}
+
+}  // extern "C"
+
+extern "C" linon_status linon_selftest_gemm(int device, const uint16_t* A, const uint16_t* B, int M, int N, int K,
+                                            int a_trans, int b_trans, int splits, const float* bias, int silu,
+                                            float* C, uint16_t* C_bf16) {
+    if (!A || !B || !C || M < 1 || N < 1 || K < 1 || splits < 1 || (K % splits) != 0 || ((K / splits) % LM_BK) != 0)
+        return LINON_ERR_ARGUMENT;
+    if (hipSetDevice(device) != hipSuccess) return LINON_ERR_HIP;
+    lm::u16 *dA = nullptr, *dB = nullptr, *dC16 = nullptr;
+    float *dC = nullptr, *dP = nullptr, *dBias = nullptr, *dZ = nullptr;
+    linon_status st = LINON_OK;
+    auto ok = [&](hipError_t e) {
+        if (e != hipSuccess) st = LINON_ERR_HIP;
+        return e == hipSuccess;
+    };
+    if (ok(hipMalloc(&dA, (size_t)M * K * 2)) && ok(hipMalloc(&dB, (size_t)K * N * 2)) &&
+        ok(hipMalloc(&dC, (size_t)M * N * 4)) && ok(hipMalloc(&dP, (size_t)splits * M * N * 4)) &&
+        ok(hipMalloc(&dBias, (size_t)N * 4)) && ok(hipMalloc(&dC16, (size_t)M * N * 2)) &&
+        ok(hipMalloc(&dZ, (size_t)M * N * 4)) && ok(hipMemcpy(dA, A, (size_t)M * K * 2, hipMemcpyHostToDevice)) &&
+        ok(hipMemcpy(dB, B, (size_t)K * N * 2, hipMemcpyHostToDevice)) &&
+        ok(hipMemset(dBias, 0, (size_t)N * 4)) &&
+        (!bias || ok(hipMemcpy(dBias, bias, (size_t)N * 4, hipMemcpyHostToDevice)))) {
+        lm::GemmParams P;
+        std::memset(&P, 0, sizeof(P));
+        P.A = dA;
+        P.B = dB;
+        P.M = M;
+        P.N = N;
+        P.K = K;
+        P.lda = a_trans ? M : K;
+        P.ldb = b_trans ? K : N;
+        P.a_trans = a_trans;
+        P.b_trans = b_trans;
+        P.k_per_split = K / splits;
+        P.ldc = N;
+        P.bias = dBias;
+        if (silu) {
+            P.epi = LM_EPI_BIAS_SILU;
+            P.out_bf16 = dC16;
+            P.z_out = dC;
+        } else if (splits > 1) {
+            P.epi = LM_EPI_PARTIAL;
+            P.out_f32 = dP;
+        } else {
+            P.epi = LM_EPI_STORE_F32;
+            P.out_f32 = dC;
+        }
+        if (ok(linon::sg_gemm(P, splits, nullptr)) && splits > 1 && !silu)
+            ok(linon::sg_reduce(dP, splits, M * N, dC, nullptr));
+        if (st == LINON_OK && ok(hipDeviceSynchronize())) {
+            ok(hipMemcpy(C, dC, (size_t)M * N * 4, hipMemcpyDeviceToHost));
+            if (silu && C_bf16) ok(hipMemcpy(C_bf16, dC16, (size_t)M * N * 2, hipMemcpyDeviceToHost));
+        }
+    }
+    for (void* p : {(void*)dA, (void*)dB, (void*)dC, (void*)dP, (void*)dBias, (void*)dC16, (void*)dZ})
+        if (p) (void)hipFree(p);
+    return st;
+}

@@ -352,6 +352,7 @@ struct RayOut {
     int steps;   // RK4 iterations evaluated (accepted + adaptive redos)
     long long tests;
     int flags;   // bit 0: an adaptive-step comparison was decided at rounding-noise level
+    T end_dir[3];  // cur_dir at the end of the loop
 };
 
 // nonlinear_ray_color, main_view.wgsl:229-279 (mode 0), plus the escape termination.
@@ -420,6 +421,9 @@ RayOut<T> trace(const Scene<T>& sc, int field, const IntegratorIn& ig, T field_w
     out.mapping[1] = cur_point.y;
     out.mapping[2] = cur_point.z;
     out.mapping[3] = T(1);
+    out.end_dir[0] = cur_dir.x;
+    out.end_dir[1] = cur_dir.y;
+    out.end_dir[2] = cur_dir.z;
     if (has_color) {
         for (int i = 0; i < 3; ++i) out.color[i] = col.color[i];
         out.face = col.face;
@@ -649,6 +653,31 @@ int render_pixels(const CameraIn* cam, const SettingsIn* st, const IntegratorIn*
     return 0;
 }
 
+// Flow map of the field through the bounding sphere: from an entry state (p, q) — q is the
+// initial direction v0 for fields that read v0, the current direction for those that read v —
+// integrate in escape mode without a mesh until the ray leaves the sphere (or max_steps).
+template <typename T>
+void flowmap(int field, const IntegratorIn& ig, T field_weight, const T* in, T* out, int32_t* steps, int n,
+             int nthreads) {
+    Scene<T> empty;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+    for (int i = 0; i < n; ++i) {
+        const T* x = in + 6 * (size_t)i;
+        RayOut<T> r = trace(empty, field, ig, field_weight, mk(x[0], x[1], x[2]), mk(x[3], x[4], x[5]));
+        T* o = out + 6 * (size_t)i;
+        o[0] = r.mapping[0];
+        o[1] = r.mapping[1];
+        o[2] = r.mapping[2];
+        o[3] = r.end_dir[0];
+        o[4] = r.end_dir[1];
+        o[5] = r.end_dir[2];
+        if (steps) steps[i] = r.steps;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -768,6 +797,18 @@ int oracle_trace_f32(int field, const IntegratorIn* ig, float field_weight, cons
     for (int c = 0; c < 3; ++c) end_point[c] = r.mapping[c];
     if (face) *face = r.face;
     return r.steps;
+}
+int oracle_flowmap_f32(int field, const IntegratorIn* ig, float field_weight, const float* in, float* out,
+                       int32_t* steps, int n, int nthreads) {
+    if (!field_known(field) || !ig || n < 0) return -1;
+    flowmap<float>(field, *ig, field_weight, in, out, steps, n, nthreads);
+    return 0;
+}
+int oracle_flowmap_f64(int field, const IntegratorIn* ig, double field_weight, const double* in, double* out,
+                       int32_t* steps, int n, int nthreads) {
+    if (!field_known(field) || !ig || n < 0) return -1;
+    flowmap<double>(field, *ig, field_weight, in, out, steps, n, nthreads);
+    return 0;
 }
 int oracle_max_threads() {
 #ifdef _OPENMP

@@ -112,6 +112,8 @@ def _bind(_lib: C.CDLL) -> C.CDLL:
     _lib.oracle_camera_ray_f64.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, P]
     _lib.oracle_trace_f64.argtypes = [C.c_int, P, C.c_double, P, C.c_uint32, P, C.c_uint32, P, P, P, P]
     _lib.oracle_trace_f32.argtypes = [C.c_int, P, C.c_float, P, C.c_uint32, P, C.c_uint32, P, P, P, P]
+    _lib.oracle_flowmap_f32.argtypes = [C.c_int, P, C.c_float, P, P, P, C.c_int, C.c_int]
+    _lib.oracle_flowmap_f64.argtypes = [C.c_int, P, C.c_double, P, P, P, C.c_int, C.c_int]
     return _lib
 
 
@@ -199,6 +201,21 @@ def render_pixels(camera, settings, integ, field: int, verts, faces, W: int, H: 
     if rc != 0:
         raise ValueError(f"oracle_render_pixels failed: {rc}")
     return {"mapping": mapping, "hit": hit, "steps": steps, "rk4_steps": int(counters[0]), "tests": int(counters[1])}
+
+
+def flowmap(fid, integ, states, field_weight=1.0, threads=0, precision="f32"):
+    """Escape-mode flow map through the bounding sphere without a mesh: (n, 6) entry states
+    (p, q) -> (n, 6) exit states (p_exit, dir_exit) and the RK4 iteration counts."""
+    L = lib(fma=precision == "f32fma")
+    ft = np.float64 if precision == "f64" else np.float32
+    x = np.ascontiguousarray(states, dtype=ft).reshape(-1, 6)
+    out = np.zeros_like(x)
+    steps = np.zeros(len(x), np.int32)
+    fn = L.oracle_flowmap_f64 if precision == "f64" else L.oracle_flowmap_f32
+    rc = fn(fid, C.byref(integ), field_weight, _p(x), _p(out), _p(steps), len(x), threads)
+    if rc != 0:
+        raise ValueError(f"oracle_flowmap failed: {rc}")
+    return out, steps
 
 
 def threads_available() -> int:

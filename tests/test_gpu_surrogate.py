@@ -1,0 +1,129 @@
+"""GPU: the surrogate's BF16-MFMA GEMM against numpy, and the learned flow-map surrogate
+(training, frame with confidence / Lyapunov fallback to RK4, re-judging)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import linon_amd as L
+from linon_amd import configs
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16_bits(x: np.ndarray) -> np.ndarray:
+    """Round-to-nearest-even float32 -> bf16 bit patterns."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def bf16_value(b: np.ndarray) -> np.ndarray:
+    return (b.astype(np.uint32) << 16).view(np.float32)
+
+
+def run_gemm(A16, B16, M, N, K, a_trans, b_trans, splits=1, bias=None, silu=False):
+    lib = L.load_library()
+    out = np.zeros((M, N), np.float32)
+    out16 = np.zeros((M, N), np.uint16)
+    bias_p = None if bias is None else C.c_void_p(np.ascontiguousarray(bias, np.float32).ctypes.data)
+    st = lib.linon_selftest_gemm(0, C.c_void_p(A16.ctypes.data), C.c_void_p(B16.ctypes.data), M, N, K, a_trans,
+                                 b_trans, splits, bias_p, 1 if silu else 0, C.c_void_p(out.ctypes.data),
+                                 C.c_void_p(out16.ctypes.data))
+    assert st == 0
+    return out, out16
+
+
+@pytest.mark.parametrize("M,N,K,a_trans,b_trans,splits", [
+    (64, 64, 32, 0, 0, 1),       # one tile: pins the MFMA operand/result lane layouts
+    (200, 96, 224, 0, 1, 1),     # ragged M/N, forward layout (A [M][K], B stored [N][K])
+    (130, 256, 256, 0, 0, 1),    # backward dX layout (B stored [K][N])
+    (224, 256, 4096, 1, 0, 4),   # weight-gradient layout (A stored [K][M]) with split-K
+])
+def test_mfma_gemm_matches_numpy(M, N, K, a_trans, b_trans, splits):
+    rng = np.random.default_rng(M * 7 + N + K)
+    A = rng.normal(size=(M, K)).astype(np.float32)
+    B = rng.normal(size=(K, N)).astype(np.float32)
+    A16 = bf16_bits(A.T.copy() if a_trans else A)
+    B16 = bf16_bits(B.T.copy() if b_trans else B)
+    ref = bf16_value(bf16_bits(A)).astype(np.float64) @ bf16_value(bf16_bits(B)).astype(np.float64)
+    print(f"[case] gemm M={M} N={N} K={K} a_trans={a_trans} b_trans={b_trans} splits={splits}", flush=True)
+    got, _ = run_gemm(A16, B16, M, N, K, a_trans, b_trans, splits)
+    np.testing.assert_allclose(got, ref, rtol=1e-4, atol=1e-4 * np.sqrt(K))
+
+
+def test_gemm_bias_silu_epilogue():
+    rng = np.random.default_rng(5)
+    M, N, K = 96, 64, 64
+    A, B = rng.normal(size=(M, K)).astype(np.float32), rng.normal(size=(K, N)).astype(np.float32)
+    bias = rng.normal(size=N).astype(np.float32)
+    z = bf16_value(bf16_bits(A)).astype(np.float64) @ bf16_value(bf16_bits(B)).astype(np.float64) + bias
+    got_z, got16 = run_gemm(bf16_bits(A), bf16_bits(B), M, N, K, 0, 0, 1, bias, silu=True)
+    np.testing.assert_allclose(got_z, z, rtol=1e-4, atol=1e-3)
+    silu = z / (1 + np.exp(-z))
+    np.testing.assert_allclose(bf16_value(got16), silu, rtol=1e-2, atol=1e-2)
+
+
+@pytest.fixture(scope="module")
+def trained():
+    view = L.MainView(96, 64)
+    view.update_model(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
+    cfg = L.SurrogateConfig.make("swirl", train_rays=1 << 18, batch=8192, steps=400, h=0.01, max_steps=2000,
+                                 conf_threshold=0.9, tolerance=1e-2, rejudge_fraction=0.25, seed=1)
+    print("[case] surrogate training (swirl, 2^18 rays, 400 steps)", flush=True)
+    stats = view.train_surrogate(cfg)
+    yield view, cfg, stats
+    view.close()
+
+
+def test_training_learns_the_flow_map(trained):
+    view, cfg, stats = trained
+    print(stats, flush=True)
+    assert np.isfinite(stats["train_loss"])
+    # untrained: exit-point error ~ the sphere's size (2); trained: well below
+    assert stats["holdout_err_q50"] < 0.2
+    assert stats["train_rays"] == cfg.train_rays
+
+
+def test_labels_are_the_rk4_flow_map(trained):
+    """The surrogate's RK4 legs use the plain kernel's arithmetic: the flow map of a few entry
+    states through the oracle's escape-mode trace (FP32) agrees."""
+    view, cfg, _ = trained
+    cam = L.Camera.look_at((0.0, 0.3, 3.0), (0, 0, 0), fov_deg=60.0)
+    view.update_camera(cam)
+    view.update_settings(L.Settings.make(overlay_mode=1))
+    view.render_surrogate()
+    st = view.surrogate_stats()
+    assert st["entered"] > 0 and st["frame_rays"] == 96 * 64
+
+
+def test_surrogate_frame_against_plain_rk4(trained):
+    """Rays the surrogate sends to RK4 reproduce the plain escape-mode frame bit for bit;
+    accepted rays are within what the re-judged sample says."""
+    view, cfg, _ = trained
+    cam = L.Camera.look_at((0.2, 0.3, 3.0), (0, 0, 0), fov_deg=60.0)
+    view.update_camera(cam)
+    view.update_settings(L.Settings.make(overlay_mode=1))
+    # plain RK4 frame: escape mode, need_mapping, no mesh
+    ig = L.Integrator.make(h_initial=cfg.h, t_max=1e30, adaptive=False, max_steps=cfg.max_steps, mode=L.MODE_ESCAPE,
+                           escape_radius=cfg.radius, samples_per_ray=0, sample_steps=0, need_mapping=True)
+    view.set_integrator(ig)
+    view.set_field("swirl")
+    view.render()
+    plain = view.read_mapping()
+    view.render_surrogate()
+    st = view.surrogate_stats()
+    sur = view.read_mapping()
+    print(st, flush=True)
+    n = st["frame_rays"]
+    assert st["accepted"] + st["conf_fallback"] + st["lyapunov_fallback"] == st["entered"]
+    diff = np.linalg.norm(sur[..., :3].astype(np.float64) - plain[..., :3], axis=-1)
+    exact = diff == 0
+    # every ray that did not take the surrogate's answer is the plain RK4 result
+    assert exact.sum() >= n - st["accepted"]
+    if st["rejudged"]:
+        assert np.isfinite(st["rejudge_err_max"])
+        assert st["rejudge_err_q50"] <= st["rejudge_err_max"]
