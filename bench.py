@@ -95,6 +95,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--batch-replays", type=int, default=2)
+    ap.add_argument("--skip-surrogate", action="store_true")
+    ap.add_argument("--surrogate-steps", type=int, default=1500)
+    ap.add_argument("--surrogate-fields", default="swirl,lens,hot_air,warp")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the ray-cast pass has no multi-GPU path in this build "
@@ -187,6 +192,14 @@ def main() -> int:
         "roofline": roofline,
     }
     view.close()
+    extra = {}
+    if not a.skip_batch:
+        log("batched views ...")
+        extra["batched_views"] = bench_batch(verts, faces, a.batch_replays)
+    if not a.skip_surrogate:
+        log("surrogate ...")
+        extra["surrogate"] = bench_surrogate(a.surrogate_steps, a.surrogate_fields.split(","))
+    out["other_workloads"] = extra
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(wl, verts, faces, a.cpu_seconds)
@@ -194,6 +207,83 @@ def main() -> int:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
     return 0
+
+
+def bench_batch(verts, faces, replays: int) -> dict:
+    """65 views (64 drawn from seed 0 + one Enhance pass at h/4) in one hipGraph replay of the
+    persistent kernels, escape mode, max 8000 RK4 steps, launch shape from the autotuner."""
+    views = configs.batch_views(num_views=64, seed=0)
+    view = L.MainView(64, 64)
+    view.update_model(verts, faces)
+    view.set_integrator(configs.batch_integrator(max_steps=8000))
+    view.set_batch(views)
+    t0 = time.perf_counter()
+    tuning, proxy_ms = view.autotune_batch(downscale=4)
+    tune_s = time.perf_counter() - t0
+    view.render_batch()
+    view.synchronize()
+    c = view.counters()
+    view.reset_timings()
+    t0 = time.perf_counter()
+    for _ in range(replays):
+        view.render_batch()
+    view.synchronize()
+    wall = (time.perf_counter() - t0) / replays
+    ms, _ = view.timings()
+    dev_ms = float(np.mean(ms))
+    rays = sum(v.width * v.height for v in views)
+    view.close()
+    return {"workload": "65 views (seed 0): resolutions {256^2..2048^2}, eye radius U(2.5,4), fields swirl/lens/"
+                        "hot-air/warp, h in {0.01,0.005,0.0025} + one Enhance view at h/4; escape mode, max 8000",
+            "views": len(views), "rays": rays, "rk4_steps": c["rk4_steps"], "device_ms": dev_ms,
+            "wall_ms": wall * 1e3, "gsteps_per_s": c["rk4_steps"] / (dev_ms * 1e-3) / 1e9,
+            "mrays_per_s": rays / (dev_ms * 1e-3) / 1e6, "refills": None,
+            "tuning": tuning.as_dict(), "autotune_proxy_ms": proxy_ms, "autotune_seconds": tune_s}
+
+
+def bench_surrogate(steps: int, fields) -> dict:
+    """Per field: train the flow-map surrogate (2^24 uniform entry states labelled by the FP32
+    RK4 kernel, batch 65536, Adam lr 1e-3), then render the 1080p view of the benchmark camera
+    without the mesh (mapping + overlay) twice: plain RK4 (escape mode) and surrogate +
+    fallback; report speed-up and the exit-point error of every pixel against the plain frame."""
+    wl = configs.CONFIG2
+    out = {}
+    for field in fields:
+        view = L.MainView(wl.width, wl.height)
+        view.update_model(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
+        view.update_camera(wl.camera())
+        view.update_settings(wl.settings())
+        cfg = L.SurrogateConfig.make(field, train_rays=1 << 24, batch=65536, steps=steps, lr=1e-3, h=wl.h,
+                                     max_steps=wl.max_steps, tolerance=1e-3, conf_threshold=0.9)
+        tr = view.train_surrogate(cfg)
+        ig = L.Integrator.make(h_initial=wl.h, t_max=1e30, adaptive=False, max_steps=wl.max_steps,
+                               mode=L.MODE_ESCAPE, escape_radius=2.0, samples_per_ray=0, sample_steps=0,
+                               need_mapping=True)
+        view.set_integrator(ig)
+        view.set_field(field)
+        view.render()
+        view.reset_timings()
+        view.render()
+        plain_ms = float(view.timings()[1][0])
+        plain = view.read_mapping()
+        view.render_surrogate()            # warm
+        view.render_surrogate()
+        st = view.surrogate_stats()
+        sur = view.read_mapping()
+        err = np.linalg.norm(sur[..., :3].astype(np.float64) - plain[..., :3], axis=-1)
+        out[field] = {"train": {k: tr[k] for k in ("train_loss", "label_seconds", "train_seconds", "holdout_err_q50",
+                                                  "holdout_err_q99", "holdout_frac_within_tol")},
+                      "plain_frame_ms": plain_ms, "surrogate_frame_ms": st["frame_ms"],
+                      "speedup": plain_ms / st["frame_ms"] if st["frame_ms"] > 0 else None,
+                      "frac_pixels_within_1e-3": float((err <= 1e-3).mean()),
+                      "rays": int(st["frame_rays"]), "entered": int(st["entered"]), "accepted": int(st["accepted"]),
+                      "conf_fallback": int(st["conf_fallback"]), "lyapunov_fallback": int(st["lyapunov_fallback"]),
+                      "rejudged": int(st["rejudged"]), "rejudge_err_q50": st["rejudge_err_q50"],
+                      "rejudge_err_q99": st["rejudge_err_q99"],
+                      "phases_ms": {k: st[k] for k in ("approach_ms", "mlp_ms", "fallback_ms", "overlay_ms")}}
+        view.close()
+        log(f"surrogate {field}: {out[field]['speedup']}x, within 1e-3: {out[field]['frac_pixels_within_1e-3']:.4f}")
+    return out
 
 
 if __name__ == "__main__":

@@ -90,6 +90,8 @@ def _bind(_lib: C.CDLL) -> C.CDLL:
     _lib.oracle_render_f64.argtypes = _lib.oracle_render_f32.argtypes
     _lib.oracle_set_prefilter.argtypes = [C.c_int]
     _lib.oracle_set_prefilter.restype = None
+    _lib.oracle_set_nudge.argtypes = [C.c_int]
+    _lib.oracle_set_nudge.restype = None
     _lib.oracle_render_pixels_f32.argtypes = [P, P, P, C.c_int, P, C.c_uint32, P, C.c_uint32, C.c_int, C.c_int,
                                               P, C.c_int, C.c_int, P, P, P, P]
     _lib.oracle_render_pixels_f64.argtypes = _lib.oracle_render_pixels_f32.argtypes
@@ -120,6 +122,32 @@ def _bind(_lib: C.CDLL) -> C.CDLL:
 def set_prefilter(on: bool) -> None:
     """Toggles the exact bounding-box prefilter of the all-faces loop (results unchanged)."""
     lib().oracle_set_prefilter(1 if on else 0)
+
+
+# FP32 variants of one restatement: "f32" (one rounding per WGSL operation, the parity
+# oracle), "f32fma" (FMA contraction allowed), "f32up" / "f32dn" (every exp, sin, cos and sqrt
+# of the field and step arithmetic moved one ulp up / down).  Their spread around the "f64"
+# result measures how far FP32 rounding alone moves a ray.
+FP32_VARIANTS = ("f32", "f32fma", "f32up", "f32dn")
+
+
+class _variant:
+    """Context manager: the oracle library of a precision name, with the ulp nudge set for
+    the duration of one call."""
+
+    def __init__(self, precision: str):
+        if precision not in FP32_VARIANTS + ("f64",):
+            raise ValueError(f"unknown precision {precision!r}")
+        self.lib = lib(fma=precision == "f32fma")
+        self.nudge = {"f32up": 1, "f32dn": -1}.get(precision, 0)
+
+    def __enter__(self):
+        self.lib.oracle_set_nudge(self.nudge)
+        return self.lib
+
+    def __exit__(self, *exc):
+        self.lib.oracle_set_nudge(0)
+        return False
 
 
 def _p(a):
@@ -162,7 +190,6 @@ def render(camera, settings, integ, field: int, verts, faces, W: int, H: int, ro
            precision: str = "f32") -> dict:
     """Ray-cast pass; returns texel-space arrays (row 0 at the top).  ``rows`` = (gy0, gy1)
     restricts the invocation rows (gy counts from the image bottom)."""
-    L = lib(fma=precision == "f32fma")
     v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
     f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
     ft = np.float64 if precision == "f64" else np.float32
@@ -174,9 +201,10 @@ def render(camera, settings, integ, field: int, verts, faces, W: int, H: int, ro
     counters = np.zeros(2, np.int64)
     flags = np.zeros((H, W), np.uint8)
     r0, r1 = rows if rows is not None else (0, H)
-    fn = L.oracle_render_f64 if precision == "f64" else L.oracle_render_f32
-    rc = fn(C.byref(camera), C.byref(settings), C.byref(integ), field, _p(v), len(v), _p(f), len(f), W, H, r0, r1,
-            threads, _p(color8), _p(color), _p(mapping), _p(hit), _p(steps), _p(counters), _p(flags))
+    with _variant(precision) as L:
+        fn = L.oracle_render_f64 if precision == "f64" else L.oracle_render_f32
+        rc = fn(C.byref(camera), C.byref(settings), C.byref(integ), field, _p(v), len(v), _p(f), len(f), W, H, r0,
+                r1, threads, _p(color8), _p(color), _p(mapping), _p(hit), _p(steps), _p(counters), _p(flags))
     if rc != 0:
         raise ValueError(f"oracle_render failed: {rc}")
     return {"color8": color8, "color": color, "mapping": mapping, "hit": hit, "steps": steps,
@@ -186,7 +214,6 @@ def render(camera, settings, integ, field: int, verts, faces, W: int, H: int, ro
 def render_pixels(camera, settings, integ, field: int, verts, faces, W: int, H: int, pixels, threads: int = 0,
                   precision: str = "f32"):
     """Rays of an explicit (n, 2) list of invocation coordinates (gx, gy)."""
-    L = lib(fma=precision == "f32fma")
     v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
     f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
     px = np.ascontiguousarray(pixels, dtype=np.int32).reshape(-1, 2)
@@ -195,9 +222,10 @@ def render_pixels(camera, settings, integ, field: int, verts, faces, W: int, H: 
     hit = np.zeros(n, np.int32)
     steps = np.zeros(n, np.int32)
     counters = np.zeros(2, np.int64)
-    fn = L.oracle_render_pixels_f64 if precision == "f64" else L.oracle_render_pixels_f32
-    rc = fn(C.byref(camera), C.byref(settings), C.byref(integ), field, _p(v), len(v), _p(f),
-            len(f), W, H, _p(px), n, threads, _p(mapping), _p(hit), _p(steps), _p(counters))
+    with _variant(precision) as L:
+        fn = L.oracle_render_pixels_f64 if precision == "f64" else L.oracle_render_pixels_f32
+        rc = fn(C.byref(camera), C.byref(settings), C.byref(integ), field, _p(v), len(v), _p(f),
+                len(f), W, H, _p(px), n, threads, _p(mapping), _p(hit), _p(steps), _p(counters))
     if rc != 0:
         raise ValueError(f"oracle_render_pixels failed: {rc}")
     return {"mapping": mapping, "hit": hit, "steps": steps, "rk4_steps": int(counters[0]), "tests": int(counters[1])}
@@ -206,13 +234,13 @@ def render_pixels(camera, settings, integ, field: int, verts, faces, W: int, H: 
 def flowmap(fid, integ, states, field_weight=1.0, threads=0, precision="f32"):
     """Escape-mode flow map through the bounding sphere without a mesh: (n, 6) entry states
     (p, q) -> (n, 6) exit states (p_exit, dir_exit) and the RK4 iteration counts."""
-    L = lib(fma=precision == "f32fma")
     ft = np.float64 if precision == "f64" else np.float32
     x = np.ascontiguousarray(states, dtype=ft).reshape(-1, 6)
     out = np.zeros_like(x)
     steps = np.zeros(len(x), np.int32)
-    fn = L.oracle_flowmap_f64 if precision == "f64" else L.oracle_flowmap_f32
-    rc = fn(fid, C.byref(integ), field_weight, _p(x), _p(out), _p(steps), len(x), threads)
+    with _variant(precision) as L:
+        fn = L.oracle_flowmap_f64 if precision == "f64" else L.oracle_flowmap_f32
+        rc = fn(fid, C.byref(integ), field_weight, _p(x), _p(out), _p(steps), len(x), threads)
     if rc != 0:
         raise ValueError(f"oracle_flowmap failed: {rc}")
     return out, steps

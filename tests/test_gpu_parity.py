@@ -9,7 +9,7 @@ import linon_amd as L
 from linon_amd import configs, scenes
 from oracle import oracle as O
 
-from gpu_common import EXIT_RTOL, HIT_MIN_AGREE, compare, rel_err, run_gpu, run_oracle, run_oracles
+from gpu_common import EXIT_RTOL, HIT_MIN_AGREE, compare, oracle_pixels, rel_err, run_gpu, run_oracle, run_oracles
 
 pytestmark = pytest.mark.gpu
 
@@ -201,11 +201,7 @@ def test_config2_full_resolution_sample(field):
     gx = rng.integers(0, wl.width, 1024)
     px = np.stack([gx, gy], 1)
     ty = wl.height - 1 - gy  # texel row of invocation row gy
-    c, s, i = (O.as_oracle_struct(wl.camera(), O.CameraIn), O.as_oracle_struct(wl.settings(), O.SettingsIn),
-               O.as_oracle_struct(wl.integrator(), O.IntegratorIn))
-    r32 = O.render_pixels(c, s, i, L.FIELDS[field], v, f, wl.width, wl.height, px, 16)
-    m64 = O.render_pixels(c, s, i, L.FIELDS[field], v, f, wl.width, wl.height, px, 16, precision="f64")["mapping"]
-    mf = O.render_pixels(c, s, i, L.FIELDS[field], v, f, wl.width, wl.height, px, 16, precision="f32fma")["mapping"]
+    r32, ref64 = oracle_pixels(wl, L.FIELDS[field], v, f, px)
     class G:
         pass
     g = G()
@@ -214,7 +210,6 @@ def test_config2_full_resolution_sample(field):
     g.ray_casting = fr.ray_casting[ty, gx][None]
     ref = {"mapping": r32["mapping"][None], "hit": r32["hit"][None], "noise_flag": np.zeros((1, len(px)), np.uint8),
            "color8": g.ray_casting}
-    ref64 = {"mapping": m64[None], "mapping_f32fma": mf[None]}
     lam = fr.lyapunov[ty, gx][None]
     info = compare(g, ref, ref64, f"config2-{field}", lam_gpu=lam)
     assert info["gated"] > 100, info

@@ -370,3 +370,25 @@ def test_prefilter_does_not_change_results():
     for k in ("hit", "mapping", "color8", "steps"):
         assert np.array_equal(res[0][k], res[1][k]), k
     assert res[1]["tests"] < res[0]["tests"]
+
+
+def test_rounding_probe_variants():
+    """The ulp-nudged FP32 variants move swirl rays (sin/cos/sqrt on every step) by rounding
+    amounts only, leave the FP64 instantiation alone and reset after each call."""
+    v, f = scenes.benchmark_scene(level=1)
+    cam = O.camera_in((0.3, 0.2, 3.0), (-0.1, -0.06, -1.0), (0, 1, 0), 60.0)
+    args = (cam, O.settings_in(), _ig(h_initial=0.01), FIELDS["swirl"], v, f, 24, 16)
+    base = O.render(*args, threads=4)
+    b64 = O.render(*args, threads=4, precision="f64")
+    for var in ("f32up", "f32dn"):
+        r = O.render(*args, threads=4, precision=var)
+        d = np.abs(r["mapping"][..., :3].astype(np.float64) - base["mapping"][..., :3]).max(-1)
+        assert (d > 0).mean() > 0.5, var           # the nudge reaches the arithmetic
+        assert np.median(d) < 1e-4, var            # and stays at rounding level
+        assert np.array_equal(O.render(*args, threads=4)["mapping"], base["mapping"])  # reset
+        assert np.array_equal(O.render(*args, threads=4, precision="f64")["mapping"], b64["mapping"])
+    up = O.render(*args, threads=4, precision="f32up")["mapping"]
+    dn = O.render(*args, threads=4, precision="f32dn")["mapping"]
+    assert not np.array_equal(up, dn)
+    with pytest.raises(ValueError):
+        O.render(*args, precision="f16")
