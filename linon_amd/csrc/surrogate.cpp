@@ -215,9 +215,11 @@ linon_status Context::surrogate_train(const linon_surrogate_config& cfg, linon_s
     LS_CHECK(this, dalloc(S.d_dz[1], (size_t)B * LS_HIDDEN));
     const int ksplit = 1024;  // batch rows per split-K chunk of a weight gradient
     const int splits = (B + ksplit - 1) / ksplit;
-    LS_CHECK(this, dalloc(S.d_part, (size_t)splits * LS_FEATS_PAD * LS_HIDDEN));
+    // one [K][LS_HIDDEN] partial per split, for the widest K of the four layers
+    const int max_k = *std::max_element(kLayerK, kLayerK + 4);
+    const size_t part_elems = (size_t)splits * max_k * LS_HIDDEN;
+    LS_CHECK(this, dalloc(S.d_part, part_elems));
     LS_CHECK(this, dalloc(S.d_loss, 1));
-    S.d_act[0] = S.d_feat;
 
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float conf_w = 0.1f;
@@ -243,6 +245,8 @@ linon_status Context::surrogate_train(const linon_surrogate_config& cfg, linon_s
         int cur = 0;
         for (int l = 3; l >= 0; --l) {
             const int K = kLayerK[l];
+            if ((size_t)splits * K * LS_HIDDEN > part_elems)
+                return fail(LINON_ERR_STATE, "surrogate: split-K partial buffer too small");
             lm::GemmParams G;  // dW = X^T dZ, split over the batch
             std::memset(&G, 0, sizeof(G));
             G.A = acts[l];
