@@ -78,7 +78,11 @@ typedef struct linon_integrator {
     int32_t samples_per_ray;  /* ray_samples records per sample ray, reference 100 */
     int32_t sample_steps;     /* RK4 steps of each sample ray, reference 5000 */
     int32_t need_mapping;     /* escape mode: keep integrating after a hit */
-    int32_t pad[3];
+    int32_t outline_rays;     /* 1: the sample rays start at the outermost pixels whose exponent
+                                 (exponents buffer of the previous render's overlay) is >= 0.8,
+                                 one per side/corner (sample_outline_rays, main_view.wgsl:381-468);
+                                 0: around the mouse position (:469-472), the reference's build */
+    int32_t pad[2];
 } linon_integrator;
 
 /* Field ids: 1..9 follow PredefinedFunction (src/functions.rs:2-13); 0 (Custom) has no
@@ -276,6 +280,11 @@ linon_status linon_read_lyapunov(linon_context* ctx, float* data, size_t bytes);
 linon_status linon_read_hit_faces(linon_context* ctx, int32_t* data, size_t bytes);
 linon_status linon_read_steps(linon_context* ctx, int32_t* data, size_t bytes);
 linon_status linon_read_ray_samples(linon_context* ctx, float* data, size_t bytes);
+/* Texel positions the last outline-ray search found (main_view.wgsl:381-468): 8 x {x, y, found}
+ * int32, x = column, y = row with row 0 at the top (the overlay's coordinates).  Not found
+ * gives {0, 0, 0}; the reference then starts that ray at pixel (0, 0) as well.  The reference
+ * keeps the positions internal; this read-back exists for tests. */
+linon_status linon_read_outline(linon_context* ctx, int32_t* data, size_t bytes);
 linon_status linon_get_counters(linon_context* ctx, linon_counters* out);
 /* Device time of every render since the last reset (the newest LINON_TIMING_SLOTS kept), from
  * HIP events recorded on the render's stream: per render the ray-cast kernel and the whole

@@ -126,15 +126,17 @@ class Integrator(C.Structure):
     _fields_ = [("h_initial", C.c_float), ("t_max", C.c_float), ("adaptive", C.c_int32),
                 ("max_steps", C.c_int32), ("mode", C.c_int32), ("escape_radius", C.c_float),
                 ("samples_per_ray", C.c_int32), ("sample_steps", C.c_int32), ("need_mapping", C.c_int32),
-                ("pad", C.c_int32 * 3)]
+                ("outline_rays", C.c_int32), ("pad", C.c_int32 * 2)]
 
     @classmethod
     def make(cls, h_initial=0.1, t_max=5.0, adaptive=True, max_steps=100000, mode=MODE_REFERENCE,
-             escape_radius=2.0, samples_per_ray=100, sample_steps=5000, need_mapping=False) -> "Integrator":
+             escape_radius=2.0, samples_per_ray=100, sample_steps=5000, need_mapping=False,
+             outline_rays=False) -> "Integrator":
         i = cls()
         i.h_initial, i.t_max, i.adaptive, i.max_steps = h_initial, t_max, int(adaptive), max_steps
         i.mode, i.escape_radius = mode, escape_radius
         i.samples_per_ray, i.sample_steps, i.need_mapping = samples_per_ray, sample_steps, int(need_mapping)
+        i.outline_rays = int(outline_rays)
         return i
 
 
@@ -242,6 +244,7 @@ SIGNATURES = {
     "linon_read_hit_faces": (C.c_int, [_P, _P, C.c_size_t]),
     "linon_read_steps": (C.c_int, [_P, _P, C.c_size_t]),
     "linon_read_ray_samples": (C.c_int, [_P, _P, C.c_size_t]),
+    "linon_read_outline": (C.c_int, [_P, _P, C.c_size_t]),
     "linon_get_counters": (C.c_int, [_P, C.POINTER(Counters)]),
     "linon_get_device_buffers": (C.c_int, [_P, C.POINTER(DeviceBuffers)]),
     "linon_get_timings": (C.c_int, [_P, _P, _P, C.c_int, C.POINTER(C.c_int)]),
@@ -433,6 +436,11 @@ class MainView:
     def read_ray_samples(self) -> np.ndarray:
         n = self.integrator().samples_per_ray
         return self._read(self._lib.linon_read_ray_samples, (8, max(1, n), 2, 4), np.float32)
+
+    def read_outline(self) -> np.ndarray:
+        """(8, 3) int32: texel (x, y) and found flag of each outline-ray search of the last
+        render with Integrator.outline_rays set."""
+        return self._read(self._lib.linon_read_outline, (8, 3), np.int32)
 
     def counters(self) -> dict:
         c = Counters()

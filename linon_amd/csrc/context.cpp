@@ -68,6 +68,7 @@ linon_status Context::alloc_images(uint32_t w, uint32_t h) {
     LN_CHECK_HIP(this, hipMemset(d_exponents, 0, n * 4));
     LN_CHECK_HIP(this, hipMemset(d_hit, 0xff, n * 4));
     LN_CHECK_HIP(this, hipMemset(d_steps, 0, n * 4));
+    if (d_outline) LN_CHECK_HIP(this, hipMemset(d_outline, 0, 8 * 8));  // positions of the old size
     return LINON_OK;
 }
 
@@ -172,6 +173,13 @@ linon_status Context::render_with(const linon_integrator& ig, hipStream_t s) {
     hipEvent_t* slot = next_timing_slot();
     for (int k = 0; k < 3; ++k) ev[k] = slot[k];
     LN_CHECK_HIP(this, hipEventRecord(ev[0], s));
+    if (ig.outline_rays && P.sample_block) {
+        // src/main_view.wgsl:381-468 reads the exponents the previous render's overlay wrote;
+        // this render's overlay runs after the ray cast, so searching first keeps that order
+        LN_CHECK_HIP(this, launch_clear_u64(d_outline, 8, s));
+        LN_CHECK_HIP(this, launch_outline_search(d_exponents, (int)width, (int)height, d_outline, s));
+        P.outline = d_outline;
+    }
     if (jit_fn) {
         void* args[] = {&P};
         LN_CHECK_HIP(this, hipModuleLaunchKernel(jit_fn, blocks, 1, 1, 64, 1, 1, (unsigned)shmem, s, args, nullptr));
@@ -276,7 +284,7 @@ linon_status linon_create(linon_context** out, int device, uint32_t width, uint3
     }
     // MainView::new defaults (src/main_view.rs:100,133-139; src/main_view.wgsl:227,241-244,287-293)
     c->settings = linon_settings{1.0f, {0.5f, 0.6f}, 0, 1, 50.0f};
-    c->integ = linon_integrator{0.1f, 5.0f, 1, 100000, LINON_MODE_REFERENCE, 2.0f, 100, 5000, 0, {0, 0, 0}};
+    c->integ = linon_integrator{0.1f, 5.0f, 1, 100000, LINON_MODE_REFERENCE, 2.0f, 100, 5000, 0, 0, {0, 0}};
     c->field_id = LINON_FIELD_MIRAGE_SPHERICAL_SIGMOID;
     // arcball default (src/arcball.rs:33-46 + zoom(-1)): eye = centre + (0,0,2) looking down -z
     std::memset(&c->camera, 0, sizeof(c->camera));
@@ -286,11 +294,14 @@ linon_status linon_create(linon_context** out, int device, uint32_t width, uint3
     for (int i = 0; i < 4; ++i) c->camera.view_matrix[5 * i] = 1.0f;
     if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         c->num_cus = 256;
-    if (hipMalloc(&c->d_counters, 8 * LINON_NUM_COUNTERS) != hipSuccess) {
+    if (hipMalloc(&c->d_counters, 8 * LINON_NUM_COUNTERS) != hipSuccess ||
+        hipMalloc(&c->d_outline, 8 * 8) != hipSuccess) {
+        if (c->d_counters) (void)hipFree(c->d_counters);
         delete c;
         return LINON_ERR_HIP;
     }
     (void)hipMemset(c->d_counters, 0, 8 * LINON_NUM_COUNTERS);
+    (void)hipMemset(c->d_outline, 0, 8 * 8);
     if (c->alloc_images(width, height) != LINON_OK || c->alloc_samples() != LINON_OK) {
         linon_destroy(reinterpret_cast<linon_context*>(c));
         return LINON_ERR_HIP;
@@ -310,6 +321,7 @@ void linon_destroy(linon_context* ctx) {
     c->batch.release();
     if (c->d_samples) (void)hipFree(c->d_samples);
     if (c->d_counters) (void)hipFree(c->d_counters);
+    if (c->d_outline) (void)hipFree(c->d_outline);
     for (auto& e : c->ring)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -480,6 +492,23 @@ linon_status linon_read_ray_samples(linon_context* ctx, float* data, size_t byte
     LN_CTX(c);
     size_t want = (size_t)8 * std::max(1, c->integ.samples_per_ray) * 32;
     return c->read(c->d_samples, data, want, bytes, "read_ray_samples");
+}
+
+linon_status linon_read_outline(linon_context* ctx, int32_t* data, size_t bytes) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    unsigned long long keys[8];
+    linon_status st = c->read(c->d_outline, keys, sizeof(keys), sizeof(keys), "read_outline");
+    if (st != LINON_OK) return st;
+    if (!data || bytes < 8 * 3 * sizeof(int32_t)) return c->fail(LINON_ERR_ARGUMENT, "read_outline: need 96 bytes");
+    for (int r = 0; r < 8; ++r) {
+        int x, y;
+        const bool found = outline_decode(keys[r], r, (int)c->width, (int)c->height, &x, &y);
+        data[3 * r] = x;
+        data[3 * r + 1] = y;
+        data[3 * r + 2] = found ? 1 : 0;
+    }
+    return LINON_OK;
 }
 
 linon_status linon_get_counters(linon_context* ctx, linon_counters* out) {

@@ -110,6 +110,7 @@ struct Context {
     float* d_samples = nullptr;
     size_t samples_bytes = 0;
     unsigned long long* d_counters = nullptr;
+    unsigned long long* d_outline = nullptr;  // 8 outline-search keys
 
     // mesh + grid
     bool have_mesh = false;

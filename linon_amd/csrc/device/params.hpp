@@ -68,7 +68,35 @@ struct RaycastParams {
     // sample rays
     float* ray_samples;   // 8 * samples_per_ray * 8 floats
     int samples_per_ray, sample_steps;
+    const unsigned long long* outline;  // 8 outline-search keys; null: rays around the mouse
 };
+
+// Outline-ray search (src/main_view.wgsl:381-468).  Each of the 8 searches scans the exponents
+// buffer (row stride H) for entries >= 0.8 and keeps either the last pixel, in x-major scan
+// order, that maximises a corner key (searches 0, 2, 4, 6: `key >= sum` lets later ties win)
+// or the first pixel of a side scan (1, 3, 5, 7).  Both become one 64-bit maximum: bit 63 =
+// found; corner searches put the key in bits 32..62 and the scan index x * H + y in the low
+// word; side searches put 0xffffffff - (scan position) in the low word.
+#define LINON_OUTLINE_MIN_EXP 0.8f
+#define LINON_OUTLINE_FOUND 0x8000000000000000ull
+
+// Texel (x, y) of search r from its key; (0, 0) when nothing was found, as in the reference.
+__host__ __device__ inline bool outline_decode(unsigned long long key, int r, int W, int H, int* x, int* y) {
+    *x = 0;
+    *y = 0;
+    if (!(key & LINON_OUTLINE_FOUND)) return false;
+    const unsigned lo = (unsigned)(key & 0xffffffffull);
+    const unsigned pos = 0xffffffffu - lo;
+    const unsigned uw = (unsigned)W, uh = (unsigned)H;
+    switch (r) {
+        case 1: *y = W - 1 - (int)(pos / uw); *x = (int)(pos % uw); break;  // y from W-1 down, x up
+        case 3: *x = W - 1 - (int)(pos / uh); *y = (int)(pos % uh); break;  // x from W-1 down, y up
+        case 5: *y = (int)(pos / uw); *x = (int)(pos % uw); break;          // y up, x up
+        case 7: *x = (int)(pos / uh); *y = (int)(pos % uh); break;          // x up, y up
+        default: *x = (int)(lo / uh); *y = (int)(lo % uh); break;           // corners: x * H + y
+    }
+    return true;
+}
 
 // One view of a batch.  Its rays occupy [first_ray, first_ray + 64 * tiles) of the batch's
 // flattened ray index space, in 8x8-tile order, so consecutive indices are neighbours.

@@ -258,8 +258,17 @@ __device__ void sample_rays(const RaycastParams& P, const CameraFrame& cam, int 
         case 6: ox = -1; oy = 1;  cr = 0.0f; cg = 0.0f; cb = 1.0f; break;
         default: ox = -1; oy = 0; cr = 0.5f; cg = 0.0f; cb = 0.5f; break;
     }
-    float px = P.mouse_x + 0.01f * ox;
-    float py = P.mouse_y + 0.01f * oy;
+    float px, py;
+    if (P.outline) {  // src/main_view.wgsl:381-468: pixel -> normalised position
+        int x, y;
+        outline_decode(P.outline[r], r, P.cam.width, P.cam.height, &x, &y);
+        const float width = (float)P.cam.width, height = (float)P.cam.height;
+        px = (float)x / (width - 1.0f);
+        py = (height - 1.0f - (float)y) / (height - 1.0f);
+    } else {
+        px = P.mouse_x + 0.01f * ox;
+        py = P.mouse_y + 0.01f * oy;
+    }
     float u2 = px * cam.vw - 0.5f * cam.vw;
     float v2 = py * cam.vh - 0.5f * cam.vh;
     vec3 s2 = u2 * cam.nh + v2 * cam.nv + 0.035f * cam.vd;
@@ -286,6 +295,47 @@ __device__ void sample_rays(const RaycastParams& P, const CameraFrame& cam, int 
         }
         p = p + step;
         t = t + h;
+    }
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// The 8 outline searches of src/main_view.wgsl:381-468 in one pass: one lane per (x, y) with
+// x < W and y < max(W, H) (search 1 runs y from W - 1 down, beyond the H rows when W > H),
+// coalesced along x; a wave maximum per search and one 64-bit atomic maximum per wave.
+// Entries the reference would read past the end of the W*H buffer read as 0 here (WGSL leaves
+// such reads to the implementation).  NaN compares false, as in WGSL.
+__device__ __forceinline__ void outline_search_body(const float* exps, int W, int H, unsigned long long* keys) {
+    const int Y = W > H ? W : H;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = t < (size_t)W * Y;
+    const int x = in ? (int)(t % (size_t)W) : 0, y = in ? (int)(t / (size_t)W) : 0;
+    const size_t idx = (size_t)y * H + x;
+    const float e = (in && idx < (size_t)W * H) ? exps[idx] : 0.0f;
+    const bool q = in && e >= LINON_OUTLINE_MIN_EXP;
+    const unsigned long long F = LINON_OUTLINE_FOUND;
+    unsigned long long k[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (q && y < H) {
+        const unsigned long long s = (unsigned long long)x * H + y;  // x-major scan index
+        k[0] = F | ((unsigned long long)((W - x) + y) << 32) | s;
+        k[2] = F | ((unsigned long long)(x + y) << 32) | s;
+        k[4] = F | ((unsigned long long)(x + (H - y)) << 32) | s;
+        k[6] = F | ((unsigned long long)((W - x) + (H - y)) << 32) | s;
+        k[3] = F | (0xffffffffull - ((unsigned long long)(W - 1 - x) * H + y));
+        k[5] = F | (0xffffffffull - ((unsigned long long)y * W + x));
+        k[7] = F | (0xffffffffull - s);
+    }
+    if (q && y < W) k[1] = F | (0xffffffffull - ((unsigned long long)(W - 1 - y) * W + x));
+    const int lane = threadIdx.x & 63;
+    for (int r = 0; r < 8; ++r) {
+        const unsigned long long m = wave_max_u64(k[r]);
+        if (lane == 0 && m) atomicMax(keys + r, m);
     }
 }
 

@@ -22,6 +22,11 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchParams B) {
 
 __global__ void __launch_bounds__(256) overlay_kernel(OverlayParams P) { lk::overlay_body(P); }
 
+__global__ void __launch_bounds__(256) outline_search_kernel(const float* exps, int W, int H,
+                                                             unsigned long long* keys) {
+    lk::outline_search_body(exps, W, H, keys);
+}
+
 __global__ void __launch_bounds__(64) clear_u64_kernel(unsigned long long* p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
@@ -89,6 +94,13 @@ hipError_t launch_overlay(const OverlayParams& P, hipStream_t s) {
     dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
     void* args[] = {const_cast<OverlayParams*>(&P)};
     return hipLaunchKernel((const void*)&overlay_kernel, grid, dim3(256), args, 0, s);
+}
+
+hipError_t launch_outline_search(const float* exps, int W, int H, unsigned long long* keys, hipStream_t s) {
+    const size_t n = (size_t)W * (size_t)(W > H ? W : H);
+    void* args[] = {&exps, &W, &H, &keys};
+    return hipLaunchKernel((const void*)&outline_search_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), args, 0,
+                           s);
 }
 
 hipError_t launch_clear_u64(unsigned long long* p, int n, hipStream_t s) {

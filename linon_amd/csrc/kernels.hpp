@@ -14,5 +14,6 @@ hipError_t launch_overlay(const OverlayParams& P, hipStream_t s);
 const void* batch_kernel_for(int id);
 hipError_t launch_batch(int field, const BatchParams& B, unsigned blocks, unsigned threads, hipStream_t s);
 hipError_t launch_clear_u64(unsigned long long* p, int n, hipStream_t s);
+hipError_t launch_outline_search(const float* exps, int W, int H, unsigned long long* keys, hipStream_t s);
 
 }  // namespace linon

@@ -316,7 +316,8 @@ struct IntegratorIn {
     int32_t samples_per_ray;
     int32_t sample_steps;
     int32_t need_mapping;  // escape mode only: keep integrating after a hit (overlay wants the end point)
-    int32_t pad[3];
+    int32_t outline_rays;  // layout only: outline positions are passed to sample_rays explicitly
+    int32_t pad[2];
 };
 
 template <typename T>
@@ -593,9 +594,57 @@ const float kSampleOffsets[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {1, 0}, {1, 1}, 
 const float kSampleColors[8][3] = {{1, 0, 0},     {0.5f, 0.5f, 0}, {0, 1, 0}, {0.5f, 1, 0},
                                    {1, 1, 0},     {0.5f, 0.5f, 0.5f}, {0, 0, 1}, {0.5f, 0, 0.5f}};
 
+// The outline-ray searches of main_view.wgsl:381-468, loop for loop: 8 scans of the exponents
+// buffer (index y * H + x, the overlay's row stride H) for entries >= 0.8.  An index at or past
+// W * H (search 1's rows y >= H, or x >= H aliasing past the end when H > W) reads as 0 here:
+// WGSL leaves out-of-bounds reads of a storage buffer to the implementation.  out: 8 x {x, y,
+// found}; not found leaves pos = (0, 0) as the WGSL does.
+void outline_search(const float* ex, int W, int H, int32_t* out) {
+    const float min_exp = 0.8f;
+    auto e = [&](int x, int y) -> float {
+        const long long idx = (long long)y * H + x;
+        return (idx >= 0 && idx < (long long)W * H) ? ex[idx] : 0.0f;
+    };
+    for (int r = 0; r < 8; ++r) {
+        int px = 0, py = 0, sum = 0;
+        bool found = false, any = false;
+        if (r == 0 || r == 2 || r == 4 || r == 6) {  // corners: key >= sum keeps the last maximum
+            for (int x = 0; x < W; ++x)
+                for (int y = 0; y < H; ++y) {
+                    int key = r == 0 ? (W - x) + y : r == 2 ? x + y : r == 4 ? x + (H - y) : (W - x) + (H - y);
+                    if (e(x, y) >= min_exp && key >= sum) {
+                        px = x;
+                        py = y;
+                        sum = key;
+                        any = true;
+                    }
+                }
+        } else if (r == 1) {  // bottom middle: y from W - 1 (sic) down, x up, first hit
+            for (int y = W - 1; y >= 0 && !found; --y)
+                for (int x = 0; x < W && !found; ++x)
+                    if (e(x, y) >= min_exp) { px = x; py = y; found = true; }
+        } else if (r == 3) {  // middle right: x from W - 1 down, y up
+            for (int x = W - 1; x >= 0 && !found; --x)
+                for (int y = 0; y < H && !found; ++y)
+                    if (e(x, y) >= min_exp) { px = x; py = y; found = true; }
+        } else if (r == 5) {  // top middle: y up, x up
+            for (int y = 0; y < H && !found; ++y)
+                for (int x = 0; x < W && !found; ++x)
+                    if (e(x, y) >= min_exp) { px = x; py = y; found = true; }
+        } else {  // middle left: x up, y up
+            for (int x = 0; x < W && !found; ++x)
+                for (int y = 0; y < H && !found; ++y)
+                    if (e(x, y) >= min_exp) { px = x; py = y; found = true; }
+        }
+        out[3 * r] = px;
+        out[3 * r + 1] = py;
+        out[3 * r + 2] = (found || any) ? 1 : 0;
+    }
+}
+
 template <typename T>
 int sample_rays(const CameraIn* cam, const SettingsIn* st, const IntegratorIn* ig, int field, int W, int H,
-                T* out) {
+                T* out, const int32_t* outline = nullptr) {
     if (!field_known(field)) return -2;
     CameraBasis<T> cb = camera_basis<T>(*cam, W, H);
     const T w = T(st->field_weight);
@@ -606,6 +655,11 @@ int sample_rays(const CameraIn* cam, const SettingsIn* st, const IntegratorIn* i
     for (int r = 0; r < 8; ++r) {
         T px = T(st->mouse_pos[0]) + T(0.01) * T(kSampleOffsets[r][0]);
         T py = T(st->mouse_pos[1]) + T(0.01) * T(kSampleOffsets[r][1]);
+        if (outline) {  // main_view.wgsl:466-467
+            const T width = T(W), height = T(H);
+            px = T(outline[3 * r]) / (width - T(1));
+            py = (height - T(1) - T(outline[3 * r + 1])) / (height - T(1));
+        }
         T u2 = px * cb.vw - T(0.5) * cb.vw;
         T v2 = py * cb.vh - T(0.5) * cb.vh;
         V3<T> s2 = u2 * cb.nh + v2 * cb.nv + cb.focal * cb.view_direction;
@@ -743,6 +797,17 @@ int oracle_sample_rays_f32(const CameraIn* cam, const SettingsIn* st, const Inte
 int oracle_sample_rays_f64(const CameraIn* cam, const SettingsIn* st, const IntegratorIn* ig, int field, int W,
                            int H, double* out) {
     return sample_rays<double>(cam, st, ig, field, W, H, out);
+}
+void oracle_outline_search(const float* exponents, int W, int H, int32_t* out) {
+    outline_search(exponents, W, H, out);
+}
+int oracle_sample_rays_outline_f32(const CameraIn* cam, const SettingsIn* st, const IntegratorIn* ig, int field,
+                                   int W, int H, const int32_t* outline, float* out) {
+    return sample_rays<float>(cam, st, ig, field, W, H, out, outline);
+}
+int oracle_sample_rays_outline_f64(const CameraIn* cam, const SettingsIn* st, const IntegratorIn* ig, int field,
+                                   int W, int H, const int32_t* outline, double* out) {
+    return sample_rays<double>(cam, st, ig, field, W, H, out, outline);
 }
 
 // ---- single-function entry points used by the pinning tests --------------------------

@@ -24,6 +24,10 @@ What pins each oracle function (tests/test_oracle_pins.py unless noted):
   lyapunov           affine mappings give a constant G whose sigma_max equals numpy's SVD
   overlay blend      closed-form alpha blend and unorm8 rounding, exponents row stride H
   sample_rays        the identity field gives straight sample rays at the documented offsets
+  outline_search     hand-worked side/corner picks, ties, the inclusive 0.8 threshold, an
+                     empty buffer, search 1's y range from W - 1 and out-of-range indices
+  f32up / f32dn      rounding probes: swirl rays move by rounding amounts only, the FP64
+                     instantiation is untouched, the nudge resets after every call
 Parity unpinned: the exact f32 bits of the reference's naga/SPIR-V/driver build (its
 transcendentals and FMA contraction), which no file of the reference pins.
 """
@@ -53,7 +57,7 @@ class IntegratorIn(C.Structure):
     _fields_ = [("h_initial", C.c_float), ("t_max", C.c_float), ("adaptive", C.c_int32),
                 ("max_steps", C.c_int32), ("mode", C.c_int32), ("escape_radius", C.c_float),
                 ("samples_per_ray", C.c_int32), ("sample_steps", C.c_int32), ("need_mapping", C.c_int32),
-                ("pad", C.c_int32 * 3)]
+                ("outline_rays", C.c_int32), ("pad", C.c_int32 * 2)]
 
 
 _lib = None
@@ -99,6 +103,10 @@ def _bind(_lib: C.CDLL) -> C.CDLL:
     _lib.oracle_overlay_f64.argtypes = _lib.oracle_overlay_f32.argtypes
     _lib.oracle_sample_rays_f32.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int, P]
     _lib.oracle_sample_rays_f64.argtypes = _lib.oracle_sample_rays_f32.argtypes
+    _lib.oracle_outline_search.argtypes = [P, C.c_int, C.c_int, P]
+    _lib.oracle_outline_search.restype = None
+    _lib.oracle_sample_rays_outline_f32.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int, P, P]
+    _lib.oracle_sample_rays_outline_f64.argtypes = _lib.oracle_sample_rays_outline_f32.argtypes
     _lib.oracle_hit_triangle_f32.argtypes = [P, P, P]
     _lib.oracle_hit_triangle_f32.restype = C.c_float
     _lib.oracle_hit_triangle_f64.argtypes = [P, P, P]
@@ -266,14 +274,33 @@ def overlay(settings, ray_casting8, mapping, precision: str = "f32") -> dict:
     return {"final": final, "exponents": expo, "lyapunov": lyap}
 
 
-def sample_rays(camera, settings, integ, field: int, W: int, H: int, precision: str = "f32") -> np.ndarray:
+def sample_rays(camera, settings, integ, field: int, W: int, H: int, precision: str = "f32",
+                outline=None) -> np.ndarray:
+    """The 8 sample rays; ``outline`` = (8, 3) int32 {x, y, found} from outline_search starts
+    them at those pixels instead of around the mouse position."""
     L = lib()
     ft = np.float32 if precision == "f32" else np.float64
     out = np.zeros((8, integ.samples_per_ray, 2, 4), ft)
-    fn = L.oracle_sample_rays_f32 if precision == "f32" else L.oracle_sample_rays_f64
-    rc = fn(C.byref(camera), C.byref(settings), C.byref(integ), field, W, H, _p(out))
+    if outline is None:
+        fn = L.oracle_sample_rays_f32 if precision == "f32" else L.oracle_sample_rays_f64
+        rc = fn(C.byref(camera), C.byref(settings), C.byref(integ), field, W, H, _p(out))
+    else:
+        xy = np.ascontiguousarray(outline, dtype=np.int32).reshape(8, 3)
+        fn = L.oracle_sample_rays_outline_f32 if precision == "f32" else L.oracle_sample_rays_outline_f64
+        rc = fn(C.byref(camera), C.byref(settings), C.byref(integ), field, W, H, _p(xy), _p(out))
     if rc != 0:
         raise ValueError(f"oracle_sample_rays failed: {rc}")
+    return out
+
+
+def outline_search(exponents, W: int, H: int) -> np.ndarray:
+    """The outline-ray searches over an exponents buffer (W * H floats, index y * H + x):
+    (8, 3) int32 {x, y, found}."""
+    e = np.ascontiguousarray(exponents, dtype=np.float32).reshape(-1)
+    if e.size != W * H:
+        raise ValueError("exponents must hold W * H values")
+    out = np.zeros((8, 3), np.int32)
+    lib().oracle_outline_search(_p(e), W, H, _p(out))
     return out
 
 

@@ -149,6 +149,38 @@ def test_sample_rays(view):
     assert pos_err.max() <= 1e-3, pos_err.max()
 
 
+@pytest.mark.parametrize("W,H", [(96, 64), (40, 72)])
+def test_outline_rays(view, W, H):
+    """Outline-ray selection (src/main_view.wgsl:381-468): the search runs on the exponents of
+    the previous render's overlay and must find exactly the oracle's pixels; the sample rays
+    then start there.  W > H and H > W cover the row-stride-H aliasing and index range."""
+    v, f, c = scenes.reference_startup_scene()
+    cam = L.Camera.arcball_default(c)
+    st = L.Settings.make(overlay_mode=1)
+    view.resize_texture(W, H)
+    print(f"[case] outline rays {W}x{H}", flush=True)
+    run_gpu(view, cam, st, L.Integrator.make(), "mirage_spherical_sigmoid", v, f)
+    exps = view.read_exponents()
+    assert (exps >= 0.8).any()
+    ig = L.Integrator.make(outline_rays=True)
+    view.set_integrator(ig)
+    view.render()
+    got = view.read_outline()
+    want = O.outline_search(exps, W, H)
+    np.testing.assert_array_equal(got, want)
+    samples = view.read_ray_samples()
+    c2, s2, i2 = (O.as_oracle_struct(cam, O.CameraIn), O.as_oracle_struct(st, O.SettingsIn),
+                  O.as_oracle_struct(ig, O.IntegratorIn))
+    ref = O.sample_rays(c2, s2, i2, 3, W, H, outline=want)
+    np.testing.assert_array_equal(samples[:, :, 1], ref[:, :, 1])
+    pos_err = np.abs(samples[:, :, 0, :3] - ref[:, :, 0, :3]) / np.maximum(1.0, np.abs(ref[:, :, 0, :3]))
+    assert pos_err.max() <= 1e-3, pos_err.max()
+    # the mouse-position rays differ from the outline rays
+    view.set_integrator(L.Integrator.make())
+    view.render()
+    assert not np.array_equal(view.read_ray_samples()[:, 1:, 0], samples[:, 1:, 0])
+
+
 def test_high_accuracy_render(view):
     """Enhance (src/main_view.rs:778-802): one render at h_initial = 0.001."""
     v, f, c = scenes.reference_startup_scene()

@@ -392,3 +392,64 @@ def test_rounding_probe_variants():
     assert not np.array_equal(up, dn)
     with pytest.raises(ValueError):
         O.render(*args, precision="f16")
+
+
+# ---------------------------------------------------------------------------------------
+# Outline-ray search (src/main_view.wgsl:381-468), expectations worked out by hand
+# ---------------------------------------------------------------------------------------
+def _exps(W, H, pixels, value=1.0):
+    e = np.zeros(W * H, np.float32)
+    for x, y in pixels:
+        e[y * H + x] = value          # the overlay's row stride H
+    return e
+
+
+def test_outline_search_sides_and_corners():
+    W = H = 10  # square: every index y * H + x names one texel
+    A, B, Cc, D = (1, 7), (8, 1), (2, 2), (7, 6)
+    got = O.outline_search(_exps(W, H, [A, B, Cc, D]), W, H)
+    # corner keys: 0 (W-x)+y -> A 16; 2 x+y -> D 13; 4 x+(H-y) -> B 17; 6 (W-x)+(H-y) -> C 16
+    # sides: 1 largest y -> A; 3 largest x -> B; 5 smallest y -> B; 7 smallest x -> A
+    want = [A, A, D, B, B, B, Cc, A]
+    assert [tuple(int(v) for v in r[:2]) for r in got] == want
+    assert (got[:, 2] == 1).all()
+
+
+def test_outline_search_row_stride_aliasing():
+    # W > H: the stride-H index of texel (10, 1) is 19 = 2 * 9 + 1, so the scans also meet it
+    # at (1, 2); each search keeps whichever alias its own order prefers
+    W, H = 12, 9
+    got = O.outline_search(_exps(W, H, [(10, 1)]), W, H)
+    a, b = (10, 1), (1, 2)
+    # keys: 0 -> b (13 vs 3); 2 -> a (11 vs 3); 4 -> a (18 vs 8); 6 -> b (18 vs 10);
+    # 1 largest y -> b; 3 largest x -> a; 5 smallest y -> a; 7 smallest x -> b
+    assert [tuple(int(v) for v in r[:2]) for r in got] == [b, b, a, a, a, a, b, b]
+
+
+def test_outline_search_ties_threshold_and_empty():
+    W = H = 12
+    # equal key 14 for search 0: (1, 3) and (3, 5); `key >= sum` keeps the later one in x-major order
+    got = O.outline_search(_exps(W, H, [(1, 3), (3, 5)]), W, H)
+    assert tuple(got[0, :2]) == (3, 5) and tuple(got[7, :2]) == (1, 3)
+    # 0.8 is inclusive, just below is not
+    assert O.outline_search(_exps(W, H, [(4, 4)], 0.8), W, H)[:, 2].all()
+    below = np.nextafter(np.float32(0.8), np.float32(0))
+    assert not O.outline_search(_exps(W, H, [(4, 4)], below), W, H)[:, 2].any()
+    none = O.outline_search(np.zeros(W * H, np.float32), W, H)
+    assert (none == 0).all()
+
+
+def test_outline_search_row_range_quirks():
+    # search 1 runs y from W - 1 (not H - 1) down: with W > H it reads rows y >= H through the
+    # stride-H index; entry 16 = 5 * 3 + 1 is only reachable that way
+    W, H = 6, 3
+    e = np.zeros(W * H, np.float32)
+    e[16] = 1.0
+    got = O.outline_search(e, W, H)
+    assert tuple(got[1]) == (1, 5, 1)
+    assert not np.delete(got[:, 2], 1).any()
+    # H > W: indices y * H + x past W * H read as 0, so with every entry set, search 0's
+    # maximum of (W - x) + y is at x = 0 on the last row whose index stays inside: y = 2
+    W, H = 3, 6
+    got = O.outline_search(np.ones(W * H, np.float32), W, H)
+    assert tuple(got[0]) == (0, 2, 1)
