@@ -100,6 +100,7 @@ def main() -> int:
     ap.add_argument("--skip-surrogate", action="store_true")
     ap.add_argument("--surrogate-steps", type=int, default=1500)
     ap.add_argument("--surrogate-fields", default="swirl,lens,hot_air,warp")
+    ap.add_argument("--skip-stress", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the ray-cast pass has no multi-GPU path in this build "
@@ -199,6 +200,9 @@ def main() -> int:
     if not a.skip_surrogate:
         log("surrogate ...")
         extra["surrogate"] = bench_surrogate(a.surrogate_steps, a.surrogate_fields.split(","))
+    if not a.skip_stress:
+        log("4K stress ...")
+        extra["stress_4k"] = bench_stress()
     out["other_workloads"] = extra
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
@@ -207,6 +211,37 @@ def main() -> int:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
     return 0
+
+
+def bench_stress() -> dict:
+    """3840x2160 at h = 0.001 (5001 RK4 iterations per ray) against the Cornell box plus the
+    1,310,720-face level-8 icosphere, lens and hot-air, overlay on, 8 sample rays x 64 records:
+    one warm-up and one timed frame per field."""
+    wl = configs.CONFIG5
+    verts, faces = wl.mesh()
+    view = L.MainView(wl.width, wl.height)
+    t0 = time.perf_counter()
+    view.update_model(verts, faces)
+    mesh_s = time.perf_counter() - t0
+    view.update_camera(wl.camera())
+    view.update_settings(wl.settings())
+    view.set_integrator(wl.integrator())
+    out = {"workload": wl.name, "faces": int(len(faces)), "mesh_upload_seconds": mesh_s, "per_field": {}}
+    for name in wl.fields:
+        view.set_field(name)
+        view.render()
+        c = view.counters()
+        view.reset_timings()
+        view.render()
+        rms, pms = view.timings()
+        fl = kernel_flops(name, c)
+        out["per_field"][name] = {"raycast_ms": float(rms[0]), "frame_ms": float(pms[0]), "rk4_steps": c["rk4_steps"],
+                                  "triangle_tests": c["triangle_tests"],
+                                  "gsteps_per_s": c["rk4_steps"] / (rms[0] * 1e-3) / 1e9,
+                                  "tflops": fl / (rms[0] * 1e-3) / 1e12}
+        log(f"stress {name}: {rms[0]:.1f} ms")
+    view.close()
+    return out
 
 
 def bench_batch(verts, faces, replays: int) -> dict:

@@ -5,6 +5,7 @@
 * ``config2``: 1920x1080, the Cornell box plus a seeded 20,480-face displaced icosphere,
   fields swirl / lens / hot-air / warp each rendered once, h = 0.005, at most 4000 RK4
   iterations, Lyapunov overlay over 3x3 neighbours (delta 1, scaling 50).
+* ``config5``: the stress case, 3840x2160 at h = 0.001 against 1.3 M faces (lens, hot-air).
 
 Both keep the reference's integration loop `for (; t <= 5.0;)` with `t = t + h` in f32
 (src/main_view.wgsl:244,265); the step cap never binds at these h.  They integrate with the
@@ -34,6 +35,7 @@ class Workload:
     t_max: float = 5.0
     adaptive: bool = False   # fixed-step RK4 ("step h = ..."); see the module docstring
     samples: bool = True
+    samples_per_ray: int = 100
 
     def camera(self) -> Camera:
         return Camera.look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), fov_deg=60.0)
@@ -45,7 +47,7 @@ class Workload:
     def integrator(self) -> Integrator:
         return Integrator.make(h_initial=self.h, t_max=self.t_max, adaptive=self.adaptive,
                                max_steps=self.max_steps, mode=MODE_REFERENCE,
-                               samples_per_ray=100 if self.samples else 0,
+                               samples_per_ray=self.samples_per_ray if self.samples else 0,
                                sample_steps=5000 if self.samples else 0)
 
     def mesh(self):
@@ -60,6 +62,11 @@ class Workload:
 CONFIG1 = Workload("plain-rk4-256-swirl-cornell", 256, 256, ("swirl",), 0.01, 1000, False, None)
 CONFIG2 = Workload("plain-rk4-overlay-1080p-4fields-cornell+ico20k", 1920, 1080,
                    ("swirl", "lens", "hot_air", "warp"), 0.005, 4000, True, 5)
+# Stress: 3840x2160, h = 0.001 (5001 RK4 iterations per ray under t <= 5), the Cornell box plus
+# the level-8 displaced icosphere (1,310,720 faces), lens and hot-air, overlay, 8 sample rays
+# recording 64 positions each.
+CONFIG5 = Workload("stress-4k-h0.001-lens+hotair-cornell+ico1.3M", 3840, 2160, ("lens", "hot_air"), 0.001,
+                   20000, True, 8, samples_per_ray=64)
 
 
 def setup_view(view, wl: Workload, field) -> None:
