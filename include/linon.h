@@ -212,6 +212,23 @@ linon_status linon_set_settings(linon_context* ctx, const linon_settings* settin
 linon_status linon_set_mesh(linon_context* ctx, const float* xyz, uint32_t num_vertices,
                             const uint32_t* faces, uint32_t num_faces);
 linon_status linon_set_accel(linon_context* ctx, int accel, int grid_resolution);
+/* Where linon_set_mesh builds the grid: host threads or device kernels, which produce the same
+ * arrays; AUTO builds on the device for meshes over 65536 faces.  Rebuilds a mesh already set.
+ * (The reference has no acceleration structure: ray_color loops over all faces,
+ * src/main_view.wgsl:177-220.) */
+typedef enum linon_grid_builder {
+    LINON_GRID_BUILD_AUTO = 0,
+    LINON_GRID_BUILD_HOST = 1,
+    LINON_GRID_BUILD_DEVICE = 2
+} linon_grid_builder;
+linon_status linon_set_grid_builder(linon_context* ctx, int builder);
+/* The grid of the current mesh (for tests; no counterpart in the reference): dims[3], the
+ * ncells + 1 cell offsets, the face ids of every cell, the ncells distance bytes, the packed
+ * faces (12 floats each) and the reference count.  Null pointers skip an array; sizes are
+ * checked against what the grid holds. */
+linon_status linon_read_grid(linon_context* ctx, int32_t* dims, uint32_t* cell_start, size_t cell_start_bytes,
+                             uint32_t* cell_faces, size_t cell_faces_bytes, uint8_t* dist, size_t dist_bytes,
+                             float* tris, size_t tris_bytes, uint64_t* refs);
 /* MainView::reload_shader with a body from PredefinedFunction::to_code (src/functions.rs:33-128). */
 linon_status linon_set_field_predefined(linon_context* ctx, int field);
 /* MainView::reload_shader(None, body) (src/main_view.rs:649-690): a WGSL field body, spliced

@@ -34,6 +34,7 @@ STATUS_NAMES = {0: "OK", 1: "ARGUMENT", 2: "COMPILE", 3: "HIP", 4: "STATE", 5: "
 MODE_REFERENCE = 0
 MODE_ESCAPE = 1
 ACCEL_AUTO, ACCEL_BRUTE, ACCEL_GRID = 0, 1, 2
+GRID_BUILD_AUTO, GRID_BUILD_HOST, GRID_BUILD_DEVICE = 0, 1, 2
 
 # Field ids (include/linon.h): 1..9 = PredefinedFunction order of src/functions.rs:2-13.
 FIELDS = {
@@ -227,6 +228,8 @@ SIGNATURES = {
     "linon_set_settings": (C.c_int, [_P, C.POINTER(Settings)]),
     "linon_set_mesh": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32]),
     "linon_set_accel": (C.c_int, [_P, C.c_int, C.c_int]),
+    "linon_set_grid_builder": (C.c_int, [_P, C.c_int]),
+    "linon_read_grid": (C.c_int, [_P, _P, _P, C.c_size_t, _P, C.c_size_t, _P, C.c_size_t, _P, C.c_size_t, _P]),
     "linon_set_field_predefined": (C.c_int, [_P, C.c_int]),
     "linon_set_field_wgsl": (C.c_int, [_P, C.c_char_p]),
     "linon_wgsl_translate": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
@@ -337,6 +340,7 @@ class MainView:
         if st != LINON_OK:
             raise LinonError(st, f"linon_create({device}, {width}, {height}) failed (is a GPU visible?)")
         self._h = h
+        self._num_faces = 0
         self.width, self.height = width, height
 
     # -- lifecycle ---------------------------------------------------------------------
@@ -377,9 +381,31 @@ class MainView:
         v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
         f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
         self._check(self._lib.linon_set_mesh(self._h, _ptr(v), len(v), _ptr(f), len(f)))
+        self._num_faces = len(f)
 
     def set_accel(self, accel: int = ACCEL_AUTO, grid_resolution: int = 0) -> None:
         self._check(self._lib.linon_set_accel(self._h, accel, grid_resolution))
+
+    def set_grid_builder(self, builder: int = GRID_BUILD_AUTO) -> None:
+        """Host threads or device kernels build the grid (same arrays); AUTO: device above
+        65536 faces.  Rebuilds the current mesh."""
+        self._check(self._lib.linon_set_grid_builder(self._h, builder))
+
+    def read_grid(self) -> dict:
+        """The current grid: dims, cell_start, cell_faces, dist and the packed faces."""
+        dims = np.zeros(3, np.int32)
+        refs = C.c_uint64(0)
+        self._check(self._lib.linon_read_grid(self._h, _ptr(dims), None, 0, None, 0, None, 0, None, 0, C.byref(refs)))
+        nc = int(np.prod(dims.astype(np.int64)))
+        out = {"dims": dims, "cell_start": np.zeros(nc + 1, np.uint32), "cell_faces": np.zeros(max(1, refs.value), np.uint32),
+               "dist": np.zeros(nc, np.uint8), "tris": np.zeros((max(1, self._num_faces), 12), np.float32)}
+        self._check(self._lib.linon_read_grid(
+            self._h, None, _ptr(out["cell_start"]), out["cell_start"].nbytes, _ptr(out["cell_faces"]),
+            out["cell_faces"].nbytes, _ptr(out["dist"]), out["dist"].nbytes, _ptr(out["tris"]), out["tris"].nbytes,
+            None))
+        out["cell_faces"] = out["cell_faces"][:refs.value]
+        out["tris"] = out["tris"][:self._num_faces]
+        return out
 
     def set_field(self, field) -> None:
         fid = FIELDS[field] if isinstance(field, str) else int(field)

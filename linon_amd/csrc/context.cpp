@@ -12,6 +12,7 @@
 #include "../../include/linon.h"
 #include "context.hpp"
 #include "grid.hpp"
+#include "grid_build.hpp"
 #include "kernels.hpp"
 
 static_assert(sizeof(linon_camera) == 112, "CameraUniform is 112 bytes (src/main_view.rs:12-19)");
@@ -223,6 +224,37 @@ linon_status Context::upload_mesh(const float* xyz, uint32_t nv, const uint32_t*
         if (faces[i] >= nv) return fail(LINON_ERR_ARGUMENT, "set_mesh: face index out of range");
     int res = grid_res_req;
     if (res == 0) res = (nf <= 65536) ? 128 : 256;
+    const bool on_device =
+        grid_builder == LINON_GRID_BUILD_DEVICE || (grid_builder == LINON_GRID_BUILD_AUTO && nf > 65536);
+    if (on_device) {
+        GridGeom geo;
+        std::string e;
+        if (!grid_geometry(xyz, nv, faces, nf, res, geo, e)) return fail(LINON_ERR_ARGUMENT, "set_mesh: " + e);
+        if (h_xyz.data() != xyz) h_xyz.assign(xyz, xyz + 3 * (size_t)nv);
+        if (h_faces.data() != faces) h_faces.assign(faces, faces + 3 * (size_t)nf);
+        LN_CHECK_HIP(this, hipSetDevice(device));
+        LN_CHECK_HIP(this, hipDeviceSynchronize());
+        free_mesh();
+        DeviceGrid dg;
+        if (!build_grid_device(geo, h_xyz.data(), nv, h_faces.data(), nf, stream, dg, e))
+            return fail(LINON_ERR_HIP, "set_mesh: " + e);
+        d_tris = dg.tris;
+        d_cell_start = dg.cell_start;
+        d_cell_faces = dg.cell_faces;
+        d_dist = dg.dist;
+        for (int k = 0; k < 3; ++k) {
+            grid_lo[k] = geo.lo[k];
+            grid_hi[k] = geo.hi[k];
+            gdim[k] = geo.dim[k];
+        }
+        cell_size = (float)geo.cs;
+        skip_margin = 4.0f * (float)geo.margin;
+        grid_refs = dg.refs;
+        num_faces = nf;
+        have_mesh = true;
+        grid_on_device = true;
+        return LINON_OK;
+    }
     GridBuild g;
     std::string e;
     if (!build_grid(xyz, nv, faces, nf, res, g, e)) return fail(LINON_ERR_ARGUMENT, "set_mesh: " + e);
@@ -254,6 +286,7 @@ linon_status Context::upload_mesh(const float* xyz, uint32_t nv, const uint32_t*
     grid_refs = g.cell_faces.size();
     num_faces = nf;
     have_mesh = true;
+    grid_on_device = false;
     return LINON_OK;
 }
 
@@ -407,6 +440,38 @@ linon_status linon_set_accel(linon_context* ctx, int accel, int grid_resolution)
     if (rebuild) return c->upload_mesh(c->h_xyz.data(), (uint32_t)(c->h_xyz.size() / 3), c->h_faces.data(),
                                        (uint32_t)(c->h_faces.size() / 3));
     return LINON_OK;
+}
+
+linon_status linon_set_grid_builder(linon_context* ctx, int builder) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    if (builder < LINON_GRID_BUILD_AUTO || builder > LINON_GRID_BUILD_DEVICE)
+        return c->fail(LINON_ERR_ARGUMENT, "set_grid_builder: unknown builder");
+    const bool rebuild = c->have_mesh && builder != c->grid_builder;
+    c->grid_builder = builder;
+    if (rebuild) return c->upload_mesh(c->h_xyz.data(), (uint32_t)(c->h_xyz.size() / 3), c->h_faces.data(),
+                                       (uint32_t)(c->h_faces.size() / 3));
+    return LINON_OK;
+}
+
+linon_status linon_read_grid(linon_context* ctx, int32_t* dims, uint32_t* cell_start, size_t cell_start_bytes,
+                             uint32_t* cell_faces, size_t cell_faces_bytes, uint8_t* dist, size_t dist_bytes,
+                             float* tris, size_t tris_bytes, uint64_t* refs) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    if (!c->have_mesh) return c->fail(LINON_ERR_STATE, "read_grid: no mesh set");
+    const size_t nc = (size_t)c->gdim[0] * c->gdim[1] * c->gdim[2];
+    if (dims)
+        for (int k = 0; k < 3; ++k) dims[k] = c->gdim[k];
+    if (refs) *refs = (uint64_t)c->grid_refs;
+    linon_status st = LINON_OK;
+    if (cell_start) st = c->read(c->d_cell_start, cell_start, (nc + 1) * 4, cell_start_bytes, "read_grid cell_start");
+    if (st == LINON_OK && cell_faces && c->grid_refs)
+        st = c->read(c->d_cell_faces, cell_faces, c->grid_refs * 4, cell_faces_bytes, "read_grid cell_faces");
+    if (st == LINON_OK && dist) st = c->read(c->d_dist, dist, nc, dist_bytes, "read_grid dist");
+    if (st == LINON_OK && tris && c->num_faces)
+        st = c->read(c->d_tris, tris, (size_t)c->num_faces * 48, tris_bytes, "read_grid tris");
+    return st;
 }
 
 linon_status linon_set_mesh(linon_context* ctx, const float* xyz, uint32_t nv, const uint32_t* faces, uint32_t nf) {

@@ -126,6 +126,8 @@ struct Context {
     size_t grid_refs = 0;
     int accel_req = LINON_ACCEL_AUTO;
     int grid_res_req = 0;
+    int grid_builder = LINON_GRID_BUILD_AUTO;
+    bool grid_on_device = false;  // how the current grid was built
     std::vector<float> h_xyz;
     std::vector<uint32_t> h_faces;
 

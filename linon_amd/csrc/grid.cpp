@@ -25,49 +25,6 @@ void pack_faces(const float* xyz, const uint32_t* faces, uint32_t nf, std::vecto
     }
 }
 
-static inline void sub3(const double* a, const double* b, double* o) {
-    o[0] = a[0] - b[0];
-    o[1] = a[1] - b[1];
-    o[2] = a[2] - b[2];
-}
-
-// Separating-axis test of a triangle against an axis-aligned box (centre c, half sizes half):
-// the 9 edge-cross-axis directions, the 3 box normals and the triangle normal.
-bool tri_box_overlap(const double c[3], const double half[3], const double v0[3], const double v1[3],
-                     const double v2[3]) {
-    double p[3][3];
-    sub3(v0, c, p[0]);
-    sub3(v1, c, p[1]);
-    sub3(v2, c, p[2]);
-    double e[3][3];
-    sub3(p[1], p[0], e[0]);
-    sub3(p[2], p[1], e[1]);
-    sub3(p[0], p[2], e[2]);
-    for (int j = 0; j < 3; ++j) {
-        const double* f = e[j];
-        const double axes[3][3] = {{0.0, -f[2], f[1]}, {f[2], 0.0, -f[0]}, {-f[1], f[0], 0.0}};
-        for (int i = 0; i < 3; ++i) {
-            const double* a = axes[i];
-            double q0 = a[0] * p[0][0] + a[1] * p[0][1] + a[2] * p[0][2];
-            double q1 = a[0] * p[1][0] + a[1] * p[1][1] + a[2] * p[1][2];
-            double q2 = a[0] * p[2][0] + a[1] * p[2][1] + a[2] * p[2][2];
-            double r = half[0] * std::fabs(a[0]) + half[1] * std::fabs(a[1]) + half[2] * std::fabs(a[2]);
-            double mn = std::min(q0, std::min(q1, q2)), mx = std::max(q0, std::max(q1, q2));
-            if (mn > r || mx < -r) return false;
-        }
-    }
-    for (int k = 0; k < 3; ++k) {
-        double mn = std::min(p[0][k], std::min(p[1][k], p[2][k]));
-        double mx = std::max(p[0][k], std::max(p[1][k], p[2][k]));
-        if (mn > half[k] || mx < -half[k]) return false;
-    }
-    double n[3] = {e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2],
-                   e[0][0] * e[1][1] - e[0][1] * e[1][0]};
-    double d = n[0] * p[0][0] + n[1] * p[0][1] + n[2] * p[0][2];
-    double r = half[0] * std::fabs(n[0]) + half[1] * std::fabs(n[1]) + half[2] * std::fabs(n[2]);
-    return std::fabs(d) <= r;
-}
-
 template <class Fn>
 static void parallel_for(size_t n, Fn fn) {
     unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -100,9 +57,8 @@ static void minmax_line(const uint8_t* in, uint8_t* out, int n, size_t s) {
     }
 }
 
-bool build_grid(const float* xyz, uint32_t nv, const uint32_t* faces, uint32_t nf, int resolution,
-                GridBuild& g, std::string& err) {
-    g = GridBuild();
+bool grid_geometry(const float* xyz, uint32_t nv, const uint32_t* faces, uint32_t nf, int resolution,
+                   GridGeom& geo, std::string& err) {
     if (resolution < 4 || resolution > 1024) {
         err = "grid resolution must be in [4, 1024]";
         return false;
@@ -137,12 +93,27 @@ bool build_grid(const float* xyz, uint32_t nv, const uint32_t* faces, uint32_t n
     for (int k = 0; k < 3; ++k) {
         double l = lo[k] - pad;
         int n = std::max(1, (int)std::ceil((hi[k] + pad - l) / cs));
-        g.dim[k] = n;
-        g.lo[k] = (float)l;
-        g.hi[k] = (float)(l + n * cs);
+        geo.dim[k] = n;
+        geo.lo[k] = (float)l;
+        geo.hi[k] = (float)(l + n * cs);
     }
-    g.cell_size = (float)cs;
-    g.margin = (float)margin;
+    geo.cs = cs;
+    geo.margin = margin;
+    return true;
+}
+
+bool build_grid(const float* xyz, uint32_t nv, const uint32_t* faces, uint32_t nf, int resolution,
+                GridBuild& g, std::string& err) {
+    g = GridBuild();
+    GridGeom geo;
+    if (!grid_geometry(xyz, nv, faces, nf, resolution, geo, err)) return false;
+    for (int k = 0; k < 3; ++k) {
+        g.dim[k] = geo.dim[k];
+        g.lo[k] = geo.lo[k];
+        g.hi[k] = geo.hi[k];
+    }
+    g.cell_size = (float)geo.cs;
+    g.margin = (float)geo.margin;
     const int nx = g.dim[0], ny = g.dim[1], nz = g.dim[2];
     const size_t nc = g.ncells();
 
@@ -151,29 +122,19 @@ bool build_grid(const float* xyz, uint32_t nv, const uint32_t* faces, uint32_t n
     std::vector<std::vector<uint64_t>> pairs(nt);
     auto emit = [&](size_t b, size_t e, unsigned tid) {
         std::vector<uint64_t>& out = pairs[tid];
-        const double half[3] = {0.5 * cs + margin, 0.5 * cs + margin, 0.5 * cs + margin};
         for (size_t f = b; f < e; ++f) {
             double v[3][3];
-            double tmin[3], tmax[3];
             for (int j = 0; j < 3; ++j)
                 for (int k = 0; k < 3; ++k) v[j][k] = xyz[3 * (size_t)faces[3 * f + j] + k];
             int c0[3], c1[3];
-            for (int k = 0; k < 3; ++k) {
-                tmin[k] = std::min(v[0][k], std::min(v[1][k], v[2][k])) - margin;
-                tmax[k] = std::max(v[0][k], std::max(v[1][k], v[2][k])) + margin;
-                c0[k] = std::max(0, (int)std::floor((tmin[k] - g.lo[k]) / cs));
-                c1[k] = std::min(g.dim[k] - 1, (int)std::floor((tmax[k] - g.lo[k]) / cs));
-            }
+            face_cell_range(geo, v, c0, c1);
             for (int z = c0[2]; z <= c1[2]; ++z)
                 for (int y = c0[1]; y <= c1[1]; ++y)
-                    for (int x = c0[0]; x <= c1[0]; ++x) {
-                        double c[3] = {g.lo[0] + (x + 0.5) * cs, g.lo[1] + (y + 0.5) * cs,
-                                       g.lo[2] + (z + 0.5) * cs};
-                        if (tri_box_overlap(c, half, v[0], v[1], v[2])) {
+                    for (int x = c0[0]; x <= c1[0]; ++x)
+                        if (face_in_cell(geo, v, x, y, z)) {
                             uint64_t cell = ((uint64_t)z * ny + y) * nx + x;
                             out.push_back((cell << 32) | (uint64_t)f);
                         }
-                    }
         }
     };
     if (nf < 4096 || nt == 1) {

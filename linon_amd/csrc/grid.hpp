@@ -4,6 +4,8 @@
 #include <string>
 #include <vector>
 
+#include "device/grid_sat.hpp"
+
 namespace linon {
 
 struct GridBuild {
@@ -21,13 +23,14 @@ struct GridBuild {
 // Per-face (v0, v1 - v0, v2 - v0) as three float4, the layout the kernels read.
 void pack_faces(const float* xyz, const uint32_t* faces, uint32_t nf, std::vector<float>& out);
 
-// Builds the grid with at most `resolution` cells along the longest axis.  Returns false and
-// sets `err` for invalid input (an index out of range, non-finite coordinates).
+// Grid geometry for a mesh: bounds of the referenced vertices padded by a margin, cubic cells
+// with at most `resolution` along the longest axis.  Validates the input (indices in range,
+// finite coordinates); returns false and sets `err` otherwise.
+bool grid_geometry(const float* xyz, uint32_t nv, const uint32_t* faces, uint32_t nf, int resolution,
+                   GridGeom& geo, std::string& err);
+
+// Builds the grid on the host (threads).  Returns false and sets `err` for invalid input.
 bool build_grid(const float* xyz, uint32_t nv, const uint32_t* faces, uint32_t nf, int resolution,
                 GridBuild& g, std::string& err);
-
-// Exposed for tests: Akenine-Moeller separating-axis triangle/box overlap (double precision).
-bool tri_box_overlap(const double c[3], const double half[3], const double v0[3], const double v1[3],
-                     const double v2[3]);
 
 }  // namespace linon
