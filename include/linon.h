@@ -249,6 +249,10 @@ linon_status linon_get_integrator(const linon_context* ctx, linon_integrator* in
 linon_status linon_render(linon_context* ctx, void* stream);
 /* MainView::render_high_accuracy (src/main_view.rs:778-802): one render at h_initial = 0.001. */
 linon_status linon_render_high_accuracy(linon_context* ctx, void* stream);
+/* MainView::render_outline_rays (src/main_view.rs:804-829): one render whose 8 sample rays
+ * start at the outermost pixels with exponent >= 0.8 of the previous render's overlay
+ * (main_view.wgsl:381-468), as integrator.outline_rays = 1 does for every render. */
+linon_status linon_render_outline_rays(linon_context* ctx, void* stream);
 linon_status linon_synchronize(linon_context* ctx);
 
 /* Batches (no counterpart in the reference, which renders one view per MainView::render).

@@ -238,6 +238,7 @@ SIGNATURES = {
     "linon_get_integrator": (C.c_int, [_P, C.POINTER(Integrator)]),
     "linon_render": (C.c_int, [_P, _P]),
     "linon_render_high_accuracy": (C.c_int, [_P, _P]),
+    "linon_render_outline_rays": (C.c_int, [_P, _P]),
     "linon_synchronize": (C.c_int, [_P]),
     "linon_read_final": (C.c_int, [_P, _P, C.c_size_t]),
     "linon_read_ray_casting": (C.c_int, [_P, _P, C.c_size_t]),
@@ -428,6 +429,9 @@ class MainView:
 
     def render_high_accuracy(self, stream: int | None = None) -> None:
         self._check(self._lib.linon_render_high_accuracy(self._h, C.c_void_p(stream)))
+
+    def render_outline_rays(self, stream: int | None = None) -> None:
+        self._check(self._lib.linon_render_outline_rays(self._h, C.c_void_p(stream)))
 
     def synchronize(self) -> None:
         self._check(self._lib.linon_synchronize(self._h))

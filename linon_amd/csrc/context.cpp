@@ -506,6 +506,14 @@ linon_status linon_render_high_accuracy(linon_context* ctx, void* stream) {
     return c->render_with(ig, reinterpret_cast<hipStream_t>(stream));
 }
 
+linon_status linon_render_outline_rays(linon_context* ctx, void* stream) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    linon_integrator ig = c->integ;
+    ig.outline_rays = 1;  // src/main_view.rs:811-814
+    return c->render_with(ig, reinterpret_cast<hipStream_t>(stream));
+}
+
 linon_status linon_synchronize(linon_context* ctx) {
     Context* c = reinterpret_cast<Context*>(ctx);
     LN_CTX(c);

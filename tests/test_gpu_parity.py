@@ -179,6 +179,11 @@ def test_outline_rays(view, W, H):
     view.set_integrator(L.Integrator.make())
     view.render()
     assert not np.array_equal(view.read_ray_samples()[:, 1:, 0], samples[:, 1:, 0])
+    # MainView::render_outline_rays: one outline render, the integrator unchanged
+    exps = view.read_exponents()
+    view.render_outline_rays()
+    np.testing.assert_array_equal(view.read_outline(), O.outline_search(exps, W, H))
+    assert view.integrator().outline_rays == 0
 
 
 def test_high_accuracy_render(view):
