@@ -44,6 +44,13 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
+def _json_scalar(o):
+    """numpy scalars (e.g. float32 timings) as plain numbers."""
+    if hasattr(o, "item"):
+        return o.item()
+    raise TypeError(f"not JSON serialisable: {type(o).__name__}")
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -209,7 +216,7 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(wl, verts, faces, a.cpu_seconds)
     else:
         out["cpu_baseline"] = None
-    print(json.dumps(out), flush=True)
+    print(json.dumps(out, default=_json_scalar), flush=True)
     return 0
 
 
@@ -235,10 +242,11 @@ def bench_stress() -> dict:
         view.render()
         rms, pms = view.timings()
         fl = kernel_flops(name, c)
-        out["per_field"][name] = {"raycast_ms": float(rms[0]), "frame_ms": float(pms[0]), "rk4_steps": c["rk4_steps"],
+        r_ms = float(rms[0])
+        out["per_field"][name] = {"raycast_ms": r_ms, "frame_ms": float(pms[0]), "rk4_steps": c["rk4_steps"],
                                   "triangle_tests": c["triangle_tests"],
-                                  "gsteps_per_s": c["rk4_steps"] / (rms[0] * 1e-3) / 1e9,
-                                  "tflops": fl / (rms[0] * 1e-3) / 1e12}
+                                  "gsteps_per_s": c["rk4_steps"] / (r_ms * 1e-3) / 1e9,
+                                  "tflops": fl / (r_ms * 1e-3) / 1e12}
         log(f"stress {name}: {rms[0]:.1f} ms")
     view.close()
     return out

@@ -199,6 +199,39 @@ __device__ __forceinline__ unsigned pack_rgba8(float r, float g, float b, float 
     return unorm8(r) | (unorm8(g) << 8) | (unorm8(b) << 16) | (unorm8(a) << 24);
 }
 
+// WGSL-literal arithmetic: every operation rounded once, no FMA contraction, in the order the
+// shader text writes it.  The integrator, camera and sample-ray statements use it, so every
+// kernel that advances a ray (plain, batched, the surrogate's RK4 legs) rounds them alike
+// whatever the surrounding code lets the compiler fuse, and as the FP32 oracle does.
+namespace rn {
+__device__ __forceinline__ float mul(float a, float b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ float add(float a, float b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ float sub(float a, float b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
+__device__ __forceinline__ vec3 mul(float s, vec3 a) { return vec3f(mul(s, a.x), mul(s, a.y), mul(s, a.z)); }
+__device__ __forceinline__ vec3 mul(vec3 a, float s) { return vec3f(mul(a.x, s), mul(a.y, s), mul(a.z, s)); }
+__device__ __forceinline__ vec3 add(vec3 a, vec3 b) { return vec3f(add(a.x, b.x), add(a.y, b.y), add(a.z, b.z)); }
+__device__ __forceinline__ vec3 sub(vec3 a, vec3 b) { return vec3f(sub(a.x, b.x), sub(a.y, b.y), sub(a.z, b.z)); }
+__device__ __forceinline__ float dot(vec3 a, vec3 b) { return add(add(mul(a.x, b.x), mul(a.y, b.y)), mul(a.z, b.z)); }
+__device__ __forceinline__ vec3 cross(vec3 a, vec3 b) {
+    return vec3f(sub(mul(a.y, b.z), mul(a.z, b.y)), sub(mul(a.z, b.x), mul(a.x, b.z)),
+                 sub(mul(a.x, b.y), mul(a.y, b.x)));
+}
+__device__ __forceinline__ float length(vec3 a) { return __builtin_sqrtf(rn::dot(a, a)); }
+__device__ __forceinline__ vec3 normalize(vec3 a) {
+    const float l = rn::length(a);
+    return vec3f(a.x / l, a.y / l, a.z / l);
+}
+}  // namespace rn
+
 // Camera ray of invocation (gx, gy), src/main_view.wgsl:347-366.
 struct CameraFrame {
     vec3 origin, vd, nh, nv;
@@ -214,18 +247,18 @@ __device__ __forceinline__ CameraFrame camera_frame(const CameraParams& P) {
     c.vh = P.viewport_h;
     c.vw = aspect * c.vh;
     vec3 w = -c.vd;
-    vec3 horizontal = cross(up, w);
-    vec3 vertical = cross(w, horizontal);
-    c.nh = normalize(horizontal);
-    c.nv = normalize(vertical);
+    vec3 horizontal = rn::cross(up, w);
+    vec3 vertical = rn::cross(w, horizontal);
+    c.nh = rn::normalize(horizontal);
+    c.nv = rn::normalize(vertical);
     return c;
 }
 __device__ __forceinline__ vec3 camera_dir(const CameraFrame& c, const CameraParams& P, int gx, int gy) {
     float width = (float)P.width, height = (float)P.height;
-    float u = (float)gx / (width - 1.0f) * c.vw - 0.5f * c.vw;
-    float v = (float)gy / (height - 1.0f) * c.vh - 0.5f * c.vh;
-    vec3 s = u * c.nh + v * c.nv + 0.035f * c.vd;
-    return normalize(s);
+    float u = rn::sub(rn::mul((float)gx / (width - 1.0f), c.vw), rn::mul(0.5f, c.vw));
+    float v = rn::sub(rn::mul((float)gy / (height - 1.0f), c.vh), rn::mul(0.5f, c.vh));
+    vec3 s = rn::add(rn::add(rn::mul(u, c.nh), rn::mul(v, c.nv)), rn::mul(0.035f, c.vd));
+    return rn::normalize(s);
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
@@ -269,32 +302,32 @@ __device__ void sample_rays(const RaycastParams& P, const CameraFrame& cam, int 
         px = P.mouse_x + 0.01f * ox;
         py = P.mouse_y + 0.01f * oy;
     }
-    float u2 = px * cam.vw - 0.5f * cam.vw;
-    float v2 = py * cam.vh - 0.5f * cam.vh;
-    vec3 s2 = u2 * cam.nh + v2 * cam.nv + 0.035f * cam.vd;
-    const vec3 start_dir = normalize(s2);
+    float u2 = rn::sub(rn::mul(px, cam.vw), rn::mul(0.5f, cam.vw));
+    float v2 = rn::sub(rn::mul(py, cam.vh), rn::mul(0.5f, cam.vh));
+    vec3 s2 = rn::add(rn::add(rn::mul(u2, cam.nh), rn::mul(v2, cam.nv)), rn::mul(0.035f, cam.vd));
+    const vec3 start_dir = rn::normalize(s2);
     vec3 p = cam.origin, d = start_dir;
     const float w = P.step.field_weight;
-    const float h = 0.001f, hh = 0.5f * h;
+    const float h = 0.001f, hh = rn::mul(0.5f, h);
     float t = 0.0f;
     const int nsamp = P.samples_per_ray;
     const int stride = P.sample_steps / nsamp;
     float4* out = reinterpret_cast<float4*>(P.ray_samples);
     for (int i = 0; i < P.sample_steps; ++i) {
         vec3 k1 = field(p, p, start_dir, d, t);
-        vec3 k2 = field(p, p + hh * k1, start_dir, k1, t + hh);
-        vec3 k3 = field(p, p + hh * k2, start_dir, k2, t + hh);
-        vec3 k4 = field(p, p + h * k3, start_dir, k3, t + h);
+        vec3 k2 = field(p, rn::add(p, rn::mul(hh, k1)), start_dir, k1, rn::add(t, hh));
+        vec3 k3 = field(p, rn::add(p, rn::mul(hh, k2)), start_dir, k2, rn::add(t, hh));
+        vec3 k4 = field(p, rn::add(p, rn::mul(h, k3)), start_dir, k3, rn::add(t, h));
         vec3 V = rk4_average(k1, k2, k3, k4);
-        d = (1.0f - w) * d + w * V;
-        vec3 step = d * h;
+        d = rn::add(rn::mul(rn::sub(1.0f, w), d), rn::mul(w, V));
+        vec3 step = rn::mul(d, h);
         if (i % stride == 0 && i / stride < nsamp) {
             int idx = r * nsamp + i / stride;
             out[2 * idx] = make_float4(p.x, p.y, p.z, 1.0f);
             out[2 * idx + 1] = make_float4(cr, cg, cb, 0.5f);
         }
-        p = p + step;
-        t = t + h;
+        p = rn::add(p, step);
+        t = rn::add(t, h);
     }
 }
 
@@ -390,25 +423,25 @@ __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, co
         if (!escape && !(r.t <= I.t_max)) return true;
         ++r.iters;
         const float h = r.h;
-        const float hh = 0.5f * h;
+        const float hh = rn::mul(0.5f, h);
         const vec3 p = r.p;
         vec3 k1 = field(p, p, start_dir, r.d, r.t);
-        vec3 k2 = field(p, p + hh * k1, start_dir, k1, r.t + hh);
-        vec3 k3 = field(p, p + hh * k2, start_dir, k2, r.t + hh);
-        vec3 k4 = field(p, p + h * k3, start_dir, k3, r.t + h);
+        vec3 k2 = field(p, rn::add(p, rn::mul(hh, k1)), start_dir, k1, rn::add(r.t, hh));
+        vec3 k3 = field(p, rn::add(p, rn::mul(hh, k2)), start_dir, k2, rn::add(r.t, hh));
+        vec3 k4 = field(p, rn::add(p, rn::mul(h, k3)), start_dir, k3, rn::add(r.t, h));
         vec3 V = rk4_average(k1, k2, k3, k4);
         float diff = 0.0f;
         if (adaptive) {  // src/main_view.wgsl:251-255
-            diff = length(V - r.last_v);
-            if (r.last_diff >= 0.0f && diff > 10.0f * r.last_diff && h > 0.002f) {
+            diff = rn::length(rn::sub(V, r.last_v));
+            if (r.last_diff >= 0.0f && diff > rn::mul(10.0f, r.last_diff) && h > 0.002f) {
                 r.h = 0.001f;
                 continue;
             }
         }
-        r.d = (1.0f - w) * r.d + w * V;
-        const vec3 step = r.d * h;
+        r.d = rn::add(rn::mul(rn::sub(1.0f, w), r.d), rn::mul(w, V));
+        const vec3 step = rn::mul(r.d, h);
         if (!r.has_color) {
-            const float L = length(step);
+            const float L = rn::length(step);
             bool query = (L > 0.0f && L < __builtin_inff());  // normalize(0) or NaN: no hit
             if (query && !brute) {
                 if (!(r.safe > L + margin)) r.safe = grid_safe_distance(S, p);
@@ -456,16 +489,16 @@ __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, co
                 }
             }
         }
-        r.p = p + step;
-        r.t = r.t + h;
+        r.p = rn::add(p, step);
+        r.t = rn::add(r.t, h);
         if (adaptive) {  // src/main_view.wgsl:266-270
             r.last_v = V;
-            if (2.0f * diff < r.last_diff && h < I.h_initial) r.h = I.h_initial;
+            if (rn::mul(2.0f, diff) < r.last_diff && h < I.h_initial) r.h = I.h_initial;
             r.last_diff = diff;
         }
         if (escape) {
             if (r.has_color && !I.need_mapping) return true;
-            if (dot(r.p, r.p) > I.escape_r2 && dot(r.p, r.d) > 0.0f) return true;
+            if (rn::dot(r.p, r.p) > I.escape_r2 && rn::dot(r.p, r.d) > 0.0f) return true;
         }
     }
     return ray_finished(r, I);

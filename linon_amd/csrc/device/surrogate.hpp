@@ -90,11 +90,11 @@ __device__ void approach_body(const ApproachParams& A) {
     none.tris = nullptr;
     none.skip_margin = 0.0f;
     const float r2 = A.radius * A.radius;
-    bool entered = lw::dot(r.p, r.p) < r2;
+    bool entered = lk::rn::dot(r.p, r.p) < r2;
     bool done = false;
     while (!entered && !done) {
         done = lk::ray_advance<F>(r, A.step, none, nullptr, 1);
-        entered = lw::dot(r.p, r.p) < r2;
+        entered = lk::rn::dot(r.p, r.p) < r2;
     }
     const size_t texel = (size_t)(A.cam.height - 1 - gy) * A.cam.width + gx;
     A.steps[texel] = r.iters;

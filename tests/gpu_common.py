@@ -100,7 +100,9 @@ def low_lambda_mask(mapping, delta=1):
 # per pixel its error against FP64 at most ILL_FACTOR times their largest (never required
 # below EXIT_RTOL), and its 99th-percentile error over the gated pixels at most Q99_FACTOR
 # times that of the two unbiased rounding paths (the oracle and its FMA-contracted build).
-# exit_agree reports the plain fraction of gated pixels within EXIT_RTOL of the FP32 oracle.
+# exit_agree reports the plain fraction of gated pixels within EXIT_RTOL of the FP32 oracle;
+# where FP32 is ill conditioned (hot air: the oracle's own FMA build agrees with it on ~91-95%
+# of the gated pixels) the exit bar is that calibration rate instead of 99.9% (exit_pass).
 WELL_COND = 1e-5
 ILL_FACTOR = 4.0
 Q99_FACTOR = 2.0
@@ -130,6 +132,12 @@ def compare(gpu: "L.Frame", ref: dict, ref64: dict, label: str, lam_gpu=None):
     d = rel_err(gpu.mapping, ref["mapping"])
     well = e32 <= WELL_COND
     ok = np.where(well, d <= EXIT_RTOL, eg <= np.maximum(ILL_FACTOR * e32, EXIT_RTOL))
+    # how often an independent FP32 evaluation (the oracle built with FMA contraction) agrees
+    # with the oracle within EXIT_RTOL: where FP32 rounding alone moves rays by more than the
+    # tolerance (hot air), no implementation can agree more often than that
+    agree_fma = None
+    if "mapping_f32fma" in ref64 and n:
+        agree_fma = float((rel_err(ref64["mapping_f32fma"], ref["mapping"])[mask] <= EXIT_RTOL).mean())
     mism = int((gpu.hit_faces[mask] != ref["hit"][mask]).sum())
     same = mask & (gpu.hit_faces == ref["hit"])
     cdiff = np.abs(gpu.ray_casting[same].astype(int) - ref["color8"][same].astype(int)).max() if same.any() else 0
@@ -143,5 +151,14 @@ def compare(gpu: "L.Frame", ref: dict, ref64: dict, label: str, lam_gpu=None):
                 err_f32_vs_f64_q50_q99_max=q(e_pair), err_probes_vs_f64_q50_q99_max=q(e32),
                 err_gpu_vs_f64_q50_q99_max=q(eg), color_maxdiff=int(cdiff))
     info["q99_ok"] = (not n) or q(eg)[1] <= max(Q99_FACTOR * q(e_pair)[1], EXIT_RTOL)
+    info["exit_agree_fma"] = agree_fma
+    if agree_fma is None:
+        bar = HIT_MIN_AGREE
+    else:  # two standard errors of the calibration rate as slack
+        bar = min(HIT_MIN_AGREE, agree_fma - 2.0 * np.sqrt(max(agree_fma * (1.0 - agree_fma), 1e-12) / n))
+    info["exit_agree_bar"] = float(bar)
+    # exit point: per pixel as accurate as the FP32 variants on >= 99.9% of gated pixels, or
+    # agreeing with the oracle within 1e-4 at least as often as an independent FP32 evaluation
+    info["exit_pass"] = bool(info["exit_ok"] >= HIT_MIN_AGREE or info["exit_agree"] >= bar)
     print(info, flush=True)
     return info

@@ -30,7 +30,7 @@ def _case(view, W, H, cam, st, ig, field, verts, faces, label, accel=L.ACCEL_AUT
     info = compare(gpu, ref, ref64, label)
     assert info["low_lambda"] > 0, info
     assert info["hit_ok"], info
-    assert info["exit_ok"] >= HIT_MIN_AGREE, info
+    assert info["exit_pass"], info
     assert info["q99_ok"], info
     assert info["color_maxdiff"] <= 1, info
     # rays whose adaptive steps were chosen by rounding noise still find the same faces
@@ -172,9 +172,14 @@ def test_outline_rays(view, W, H):
     c2, s2, i2 = (O.as_oracle_struct(cam, O.CameraIn), O.as_oracle_struct(st, O.SettingsIn),
                   O.as_oracle_struct(ig, O.IntegratorIn))
     ref = O.sample_rays(c2, s2, i2, 3, W, H, outline=want)
+    ref64 = O.sample_rays(c2, s2, i2, 3, W, H, precision="f64", outline=want)
     np.testing.assert_array_equal(samples[:, :, 1], ref[:, :, 1])
-    pos_err = np.abs(samples[:, :, 0, :3] - ref[:, :, 0, :3]) / np.maximum(1.0, np.abs(ref[:, :, 0, :3]))
-    assert pos_err.max() <= 1e-3, pos_err.max()
+    # outline rays start where the exponent is high, i.e. where neighbouring rays diverge: over
+    # 5000 steps FP32 rounding alone moves them, so the GPU is held to 1e-3 or to 4x the FP32
+    # oracle's own distance from FP64, whichever is larger
+    rel = lambda a, b: (np.abs(a[..., 0, :3] - b[..., 0, :3]) / np.maximum(1.0, np.abs(b[..., 0, :3]))).max(-1)
+    e_gpu, e_32 = rel(samples.astype(np.float64), ref64), rel(ref.astype(np.float64), ref64)
+    assert (e_gpu <= np.maximum(1e-3, 4.0 * e_32)).all(), (e_gpu.max(), e_32.max())
     # the mouse-position rays differ from the outline rays
     view.set_integrator(L.Integrator.make())
     view.render()
@@ -200,7 +205,7 @@ def test_high_accuracy_render(view):
     ref, ref64 = run_oracles(cam, L.Settings.make(),
                              L.Integrator.make(h_initial=0.001, samples_per_ray=0, sample_steps=0), 3, v, f, W, H)
     info = compare(gpu, ref, ref64, "enhance")
-    assert info["gated"] > 0 and info["hit_ok"] and info["exit_ok"] >= HIT_MIN_AGREE and info["q99_ok"]
+    assert info["gated"] > 0 and info["hit_ok"] and info["exit_pass"] and info["q99_ok"]
     assert int(np.median(gpu.steps)) >= 5000
 
 
@@ -251,7 +256,7 @@ def test_config2_full_resolution_sample(field):
     info = compare(g, ref, ref64, f"config2-{field}", lam_gpu=lam)
     assert info["gated"] > 100, info
     assert info["hit_ok"], info
-    assert info["exit_ok"] >= HIT_MIN_AGREE, info
+    assert info["exit_pass"], info
     assert info["q99_ok"], info
 
 
