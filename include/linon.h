@@ -229,6 +229,21 @@ linon_status linon_set_grid_builder(linon_context* ctx, int builder);
 linon_status linon_read_grid(linon_context* ctx, int32_t* dims, uint32_t* cell_start, size_t cell_start_bytes,
                              uint32_t* cell_faces, size_t cell_faces_bytes, uint8_t* dist, size_t dist_bytes,
                              float* tris, size_t tris_bytes, uint64_t* refs);
+/* Arithmetic of the ray-cast kernels.  The reference's precision is the Vulkan driver's, within
+ * the bounds WGSL states (division 2.5 ulp, sqrt from inverseSqrt, sin/cos 2^-11 absolute on
+ * [-pi, pi], exp 3 + 2|x| ulp).  IEEE: correctly rounded division and sqrt, full-precision
+ * transcendentals.  WGSL: hardware division, sqrt and transcendentals within those bounds.
+ * AUTO (default): WGSL for the predefined fields whose end points measure as accurate under it
+ * (translations, rotation, Lorenz, Roessler, identity, swirl, lens, warp), IEEE for the
+ * refraction fields (the four mirages, hot air), where the eta*cos - sqrt(k) cancellation near
+ * grazing incidence amplifies division rounding, and for WGSL bodies.  (No counterpart in the
+ * reference; its driver decides.) */
+typedef enum linon_math {
+    LINON_MATH_AUTO = 0,
+    LINON_MATH_IEEE = 1,
+    LINON_MATH_WGSL = 2
+} linon_math;
+linon_status linon_set_math(linon_context* ctx, int math);
 /* MainView::reload_shader with a body from PredefinedFunction::to_code (src/functions.rs:33-128). */
 linon_status linon_set_field_predefined(linon_context* ctx, int field);
 /* MainView::reload_shader(None, body) (src/main_view.rs:649-690): a WGSL field body, spliced

@@ -35,6 +35,7 @@ MODE_REFERENCE = 0
 MODE_ESCAPE = 1
 ACCEL_AUTO, ACCEL_BRUTE, ACCEL_GRID = 0, 1, 2
 GRID_BUILD_AUTO, GRID_BUILD_HOST, GRID_BUILD_DEVICE = 0, 1, 2
+MATH_AUTO, MATH_IEEE, MATH_WGSL = 0, 1, 2
 
 # Field ids (include/linon.h): 1..9 = PredefinedFunction order of src/functions.rs:2-13.
 FIELDS = {
@@ -229,6 +230,7 @@ SIGNATURES = {
     "linon_set_mesh": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32]),
     "linon_set_accel": (C.c_int, [_P, C.c_int, C.c_int]),
     "linon_set_grid_builder": (C.c_int, [_P, C.c_int]),
+    "linon_set_math": (C.c_int, [_P, C.c_int]),
     "linon_read_grid": (C.c_int, [_P, _P, _P, C.c_size_t, _P, C.c_size_t, _P, C.c_size_t, _P, C.c_size_t, _P]),
     "linon_set_field_predefined": (C.c_int, [_P, C.c_int]),
     "linon_set_field_wgsl": (C.c_int, [_P, C.c_char_p]),
@@ -386,6 +388,10 @@ class MainView:
 
     def set_accel(self, accel: int = ACCEL_AUTO, grid_resolution: int = 0) -> None:
         self._check(self._lib.linon_set_accel(self._h, accel, grid_resolution))
+
+    def set_math(self, math: int = MATH_AUTO) -> None:
+        """IEEE, WGSL-accuracy or per-field AUTO arithmetic for the ray-cast kernels."""
+        self._check(self._lib.linon_set_math(self._h, math))
 
     def set_grid_builder(self, builder: int = GRID_BUILD_AUTO) -> None:
         """Host threads or device kernels build the grid (same arrays); AUTO: device above

@@ -127,7 +127,8 @@ linon_status Context::batch_launch(BatchState& B, const linon_batch_tuning& T, h
         P.compact_every = std::max(1, T.compact_every);
         P.refill_min = std::max(1, std::min(64, T.refill_min));
         P.scene = S;
-        LB_CHECK(this, launch_batch(g.field, P, blocks, 64u * (unsigned)wpb, s));
+        LB_CHECK(this, wgsl_math_for(g.field) ? linon_wgslmath::launch_batch(g.field, P, blocks, 64u * (unsigned)wpb, s)
+                                               : launch_batch(g.field, P, blocks, 64u * (unsigned)wpb, s));
     }
     return LINON_OK;
 }

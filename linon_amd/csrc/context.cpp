@@ -185,7 +185,9 @@ linon_status Context::render_with(const linon_integrator& ig, hipStream_t s) {
         void* args[] = {&P};
         LN_CHECK_HIP(this, hipModuleLaunchKernel(jit_fn, blocks, 1, 1, 64, 1, 1, (unsigned)shmem, s, args, nullptr));
     } else {
-        LN_CHECK_HIP(this, launch_raycast_static(field_id, P, blocks, shmem, s));
+        LN_CHECK_HIP(this, wgsl_math_for(field_id)
+                               ? linon_wgslmath::launch_raycast_static(field_id, P, blocks, shmem, s)
+                               : launch_raycast_static(field_id, P, blocks, shmem, s));
     }
     LN_CHECK_HIP(this, hipEventRecord(ev[1], s));
     OverlayParams O;
@@ -205,6 +207,19 @@ linon_status Context::render_with(const linon_integrator& ig, hipStream_t s) {
     LN_CHECK_HIP(this, hipEventRecord(ev[2], s));
     timed = true;
     return LINON_OK;
+}
+
+bool Context::wgsl_math_for(int field) const {
+    if (math == LINON_MATH_IEEE) return false;
+    if (math == LINON_MATH_WGSL) return true;
+    switch (field) {  // AUTO
+        case LINON_FIELD_TRANSLATION_X: case LINON_FIELD_TRANSLATION_Z: case LINON_FIELD_ROTATION:
+        case LINON_FIELD_LORENZ: case LINON_FIELD_ROESSLER: case LINON_FIELD_IDENTITY:
+        case LINON_FIELD_SWIRL: case LINON_FIELD_LENS: case LINON_FIELD_WARP:
+            return true;
+        default:  // mirages, hot air: refraction
+            return false;
+    }
 }
 
 linon_status Context::read(const void* src, void* dst, size_t want, size_t bytes, const char* what) {
@@ -439,6 +454,20 @@ linon_status linon_set_accel(linon_context* ctx, int accel, int grid_resolution)
     c->grid_res_req = grid_resolution;
     if (rebuild) return c->upload_mesh(c->h_xyz.data(), (uint32_t)(c->h_xyz.size() / 3), c->h_faces.data(),
                                        (uint32_t)(c->h_faces.size() / 3));
+    return LINON_OK;
+}
+
+linon_status linon_set_math(linon_context* ctx, int math) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    if (math < LINON_MATH_AUTO || math > LINON_MATH_WGSL) return c->fail(LINON_ERR_ARGUMENT, "set_math: unknown mode");
+    c->math = math;
+    // a WGSL body is compiled for one build: recompile it for the new one
+    if (c->jit_fn && c->jit_wgsl_math != (math == LINON_MATH_WGSL)) {
+        const std::string body = c->jit_body;
+        return linon_set_field_wgsl(ctx, body.c_str());
+    }
+    if (c->batch.exec) return c->batch_capture();
     return LINON_OK;
 }
 

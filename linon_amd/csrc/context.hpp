@@ -95,6 +95,9 @@ struct Context {
 
     // field: a predefined id, or a kernel compiled at run time from a WGSL body
     int field_id = -1;
+    int math = LINON_MATH_AUTO;
+    bool wgsl_math_for(int field) const;  // which kernel build a predefined field runs with
+    bool jit_wgsl_math = false;           // the build the current WGSL body was compiled as
     hipModule_t jit_module = nullptr;
     hipFunction_t jit_fn = nullptr;
     std::string jit_body;

@@ -112,21 +112,33 @@ __device__ __forceinline__ vec3 swizzle3(vec3 v) {
 // ---- helpers of src/main_view.wgsl:76-146 ----------------------------------------------
 constexpr float PI = 3.141592653589793f;
 
+// sin and cos of one angle.  The WGSL-accuracy build (LINON_FAST_MATH, compiled with
+// -fgpu-approx-transcendentals) uses the hardware sine/cosine, within WGSL's stated bound
+// (absolute error 2^-11 on [-pi, pi]); the IEEE build OCML's full-precision sincosf.
+__device__ __forceinline__ void sin_cos(float x, float* s, float* c) {
+#ifdef LINON_FAST_MATH
+    *s = __sinf(x);
+    *c = __cosf(x);
+#else
+    ::sincosf(x, s, c);
+#endif
+}
+
 // The three rotations multiply by mat3x3 columns as the WGSL text lays them out; rotateZ's
 // third column is zero, so its z component is always 0 (src/main_view.wgsl:94-100).
 __device__ __forceinline__ vec3 rotateX(vec3 v, float phi) {
     float s, c;
-    ::sincosf(phi, &s, &c);
+    sin_cos(phi, &s, &c);
     return {v.x, c * v.y - s * v.z, s * v.y + c * v.z};
 }
 __device__ __forceinline__ vec3 rotateY(vec3 v, float phi) {
     float s, c;
-    ::sincosf(phi, &s, &c);
+    sin_cos(phi, &s, &c);
     return {c * v.x + s * v.z, v.y, -s * v.x + c * v.z};
 }
 __device__ __forceinline__ vec3 rotateZ(vec3 v, float phi) {
     float s, c;
-    ::sincosf(phi, &s, &c);
+    sin_cos(phi, &s, &c);
     return {c * v.x - s * v.y, s * v.x + c * v.y, 0.0f};
 }
 __device__ __forceinline__ vec3 translate(vec3 v, float dx, float dy, float dz) {

@@ -30,11 +30,15 @@ std::map<std::string, std::vector<char>> g_code_cache;  // translated source -> 
 // that can fall off its end without returning is an error, as it is for naga.
 const char* kOptions[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Werror=return-type",
                           "-Wno-unused-parameter", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+// added for the WGSL-accuracy build (linon_set_math(LINON_MATH_WGSL))
+const char* kWgslMathOptions[] = {"-DLINON_FAST_MATH", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+                                  "-fgpu-approx-transcendentals"};
 
-bool compile_source(const std::string& src, std::vector<char>& code, std::string& log) {
+bool compile_source(const std::string& src, bool wgsl_math, std::vector<char>& code, std::string& log) {
+    const std::string key = (wgsl_math ? "W|" : "I|") + src;
     {
         std::lock_guard<std::mutex> lk(g_cache_mu);
-        auto it = g_code_cache.find(src);
+        auto it = g_code_cache.find(key);
         if (it != g_code_cache.end()) {
             code = it->second;
             return true;
@@ -45,7 +49,9 @@ bool compile_source(const std::string& src, std::vector<char>& code, std::string
         log = "hiprtcCreateProgram failed";
         return false;
     }
-    hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(kOptions) / sizeof(kOptions[0])), kOptions);
+    std::vector<const char*> opts(kOptions, kOptions + sizeof(kOptions) / sizeof(kOptions[0]));
+    if (wgsl_math) opts.insert(opts.end(), kWgslMathOptions, kWgslMathOptions + 3);
+    hiprtcResult rc = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
     size_t log_size = 0;
     hiprtcGetProgramLogSize(prog, &log_size);
     if (log_size > 1) {
@@ -65,11 +71,12 @@ bool compile_source(const std::string& src, std::vector<char>& code, std::string
     hiprtcGetCode(prog, code.data());
     hiprtcDestroyProgram(&prog);
     std::lock_guard<std::mutex> lk(g_cache_mu);
-    g_code_cache[src] = code;
+    g_code_cache[key] = code;
     return true;
 }
 
-bool build_user_kernel(const char* body, std::vector<char>& code, std::string& msg, std::string* translated_out) {
+bool build_user_kernel(const char* body, bool wgsl_math, std::vector<char>& code, std::string& msg,
+                       std::string* translated_out) {
     std::string translated, err;
     if (!translate_wgsl_body(body, translated, err)) {
         msg = err;
@@ -78,7 +85,7 @@ bool build_user_kernel(const char* body, std::vector<char>& code, std::string& m
     if (translated_out) *translated_out = translated;
     std::string src = std::string(linon_device_source) + "\n" + user_field_source(translated);
     std::string log;
-    if (!compile_source(src, code, log)) {
+    if (!compile_source(src, wgsl_math, code, log)) {
         msg = "hiprtc: " + log;
         return false;
     }
@@ -112,7 +119,8 @@ linon_status linon_set_field_wgsl(linon_context* ctx, const char* body) {
     if (!body) return c->fail(LINON_ERR_ARGUMENT, "set_field_wgsl: null body");
     std::vector<char> code;
     std::string msg;
-    if (!linon::build_user_kernel(body, code, msg, nullptr)) return c->fail(LINON_ERR_COMPILE, msg);
+    const bool wgsl_math = c->math == LINON_MATH_WGSL;  // AUTO compiles WGSL bodies with IEEE arithmetic
+    if (!linon::build_user_kernel(body, wgsl_math, code, msg, nullptr)) return c->fail(LINON_ERR_COMPILE, msg);
     if (hipSetDevice(c->device) != hipSuccess) return c->fail(LINON_ERR_HIP, "set_field_wgsl: hipSetDevice");
     hipModule_t mod = nullptr;
     hipFunction_t fn = nullptr;
@@ -129,6 +137,7 @@ linon_status linon_set_field_wgsl(linon_context* ctx, const char* body) {
     c->jit_module = mod;
     c->jit_fn = fn;
     c->jit_body = body;
+    c->jit_wgsl_math = wgsl_math;
     c->field_id = -1;
     return LINON_OK;
 }
@@ -150,7 +159,7 @@ linon_status linon_wgsl_compile_check(const char* body, char* log, size_t capaci
     if (!body) return LINON_ERR_ARGUMENT;
     std::vector<char> code;
     std::string msg;
-    if (!linon::build_user_kernel(body, code, msg, nullptr)) {
+    if (!linon::build_user_kernel(body, false, code, msg, nullptr)) {
         linon::copy_out(msg, log, capacity);
         return LINON_ERR_COMPILE;
     }

@@ -8,7 +8,7 @@
 #include "device/raycast.hpp"
 #include "kernels.hpp"
 
-namespace linon {
+namespace LINON_KERNEL_NS {
 
 template <class F>
 __global__ void __launch_bounds__(64) raycast_kernel(RaycastParams P) {
@@ -108,4 +108,4 @@ hipError_t launch_clear_u64(unsigned long long* p, int n, hipStream_t s) {
     return hipLaunchKernel((const void*)&clear_u64_kernel, dim3((n + 63) / 64), dim3(64), args, 0, s);
 }
 
-}  // namespace linon
+}  // namespace LINON_KERNEL_NS

@@ -198,7 +198,8 @@ linon_status Context::surrogate_train(const linon_surrogate_config& cfg, linon_s
     F.states = S.d_states;
     F.n = N;
     F.exits = S.d_targets;
-    LS_CHECK(this, sg_flowmap(cfg.field, F, s));
+    LS_CHECK(this, wgsl_math_for(cfg.field) ? linon_wgslmath::sg_flowmap(cfg.field, F, s)
+                                              : sg_flowmap(cfg.field, F, s));
     LS_CHECK(this, hipStreamSynchronize(s));
     auto t1 = std::chrono::steady_clock::now();
 
@@ -378,7 +379,7 @@ linon_status Context::surrogate_eval(const float* states, const float* targets, 
         for (size_t i = 0; i < (size_t)n; ++i)
             for (int k = 0; k < 3; ++k) t[6 * i + k] /= S.cfg.radius;
         if (hipMemcpy(scaled, t.data(), t.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemset(loss, 0, 4) != hipSuccess) {
+            hipMemset(loss, 0, 4) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
             st = fail(LINON_ERR_HIP, "eval copy");
             break;
         }
@@ -387,7 +388,9 @@ linon_status Context::surrogate_eval(const float* states, const float* targets, 
             break;
         }
         err.resize(n);
-        if (hipMemcpy(err.data(), e, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        // the kernels ran on the context's (non-blocking) stream: wait for them before copying
+        if (hipStreamSynchronize(s) != hipSuccess ||
+            hipMemcpy(err.data(), e, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess)
             st = fail(LINON_ERR_HIP, "eval copy");
     } while (false);
     (void)hipDeviceSynchronize();
@@ -446,7 +449,8 @@ linon_status Context::surrogate_frame(hipStream_t s) {
     A.mapping = d_mapping;
     A.steps = d_steps;
     A.q_is_v0 = q_is_v0 ? 1 : 0;
-    LS_CHECK(this, sg_approach(S.cfg.field, A, s));
+    LS_CHECK(this, wgsl_math_for(S.cfg.field) ? linon_wgslmath::sg_approach(S.cfg.field, A, s)
+                                                : sg_approach(S.cfg.field, A, s));
     LS_CHECK(this, sg_fill_i32(S.f_count, 1, 0, s));
     LS_CHECK(this, sg_compact_entered(S.f_entered, S.f_entry, npx, S.f_states, S.f_texel, S.f_count, s));
     int n_in = 0;
@@ -481,7 +485,8 @@ linon_status Context::surrogate_frame(hipStream_t s) {
     if (n_fb > 0) {
         F.index = S.f_list;
         F.n = n_fb;
-        LS_CHECK(this, sg_flowmap(S.cfg.field, F, s));
+        LS_CHECK(this, wgsl_math_for(S.cfg.field) ? linon_wgslmath::sg_flowmap(S.cfg.field, F, s)
+                                                    : sg_flowmap(S.cfg.field, F, s));
         LS_CHECK(this, sg_apply_rk4(S.f_list, n_fb, S.f_exits, S.f_texel, S.f_pred, d_mapping, nullptr, 1, s));
     }
     LS_CHECK(this, hipEventRecord(e[3], s));
@@ -511,7 +516,8 @@ linon_status Context::surrogate_frame(hipStream_t s) {
         if (n_ly > 0) {
             F.index = S.f_list;
             F.n = n_ly;
-            LS_CHECK(this, sg_flowmap(S.cfg.field, F, s));
+            LS_CHECK(this, wgsl_math_for(S.cfg.field) ? linon_wgslmath::sg_flowmap(S.cfg.field, F, s)
+                                                    : sg_flowmap(S.cfg.field, F, s));
             LS_CHECK(this, sg_apply_rk4(S.f_list, n_ly, S.f_exits, S.f_texel, S.f_pred, d_mapping, nullptr, 1, s));
             LS_CHECK(this, launch_overlay(O, s));
         }
@@ -529,7 +535,8 @@ linon_status Context::surrogate_frame(hipStream_t s) {
         if (n_rj > 0) {
             F.index = S.f_list;
             F.n = n_rj;
-            LS_CHECK(this, sg_flowmap(S.cfg.field, F, s));
+            LS_CHECK(this, wgsl_math_for(S.cfg.field) ? linon_wgslmath::sg_flowmap(S.cfg.field, F, s)
+                                                    : sg_flowmap(S.cfg.field, F, s));
             LS_CHECK(this, sg_apply_rk4(S.f_list, n_rj, S.f_exits, S.f_texel, S.f_pred, d_mapping, S.f_errs, 0, s));
             rj.resize(n_rj);
             LS_CHECK(this, hipMemcpyAsync(rj.data(), S.f_errs, (size_t)n_rj * 4, hipMemcpyDeviceToHost, s));

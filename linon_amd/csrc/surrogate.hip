@@ -8,7 +8,7 @@
 #include "device/surrogate.hpp"
 #include "surrogate_kernels.hpp"
 
-namespace linon {
+namespace LINON_KERNEL_NS {
 
 using lm::u16;
 
@@ -413,4 +413,4 @@ hipError_t sg_flowmap(int field, const ls::FlowmapParams& P, hipStream_t s) {
     return hipLaunchKernel(fn, dim3((P.n + 255) / 256), dim3(256), args, 0, s);
 }
 
-}  // namespace linon
+}  // namespace LINON_KERNEL_NS

@@ -56,6 +56,21 @@ def test_config_camera_cornell(view, field):
     _case(view, 64, 64, wl.camera(), wl.settings(), wl.integrator(), field, v, f, f"cornell-{field}")
 
 
+@pytest.mark.parametrize("math", ["ieee", "wgsl"])
+@pytest.mark.parametrize("field", ["swirl", "lens", "warp"])
+def test_math_builds(view, field, math):
+    """Both kernel builds against the oracle: IEEE (correctly rounded division and sqrt,
+    full-precision transcendentals) and WGSL accuracy (hardware division, sqrt, sin/cos/exp
+    within WGSL's bounds), which AUTO selects for these fields."""
+    wl = configs.CONFIG1
+    v, f = wl.mesh()
+    view.set_math({"ieee": L.MATH_IEEE, "wgsl": L.MATH_WGSL}[math])
+    try:
+        _case(view, 64, 64, wl.camera(), wl.settings(), wl.integrator(), field, v, f, f"cornell-{field}-{math}")
+    finally:
+        view.set_math(L.MATH_AUTO)
+
+
 @pytest.mark.parametrize("field", ["swirl", "lens", "hot_air", "warp"])
 def test_benchmark_scene_small(view, field):
     wl = configs.CONFIG2
@@ -267,13 +282,18 @@ def test_wgsl_body_matches_predefined_functor(view, field):
     wl = configs.CONFIG2
     v, f = scenes.benchmark_scene(level=2)
     view.resize_texture(64, 40)
-    print(f"[case] wgsl-vs-functor {field}", flush=True)
-    a = run_gpu(view, wl.camera(), wl.settings(), wl.integrator(), field, v, f)
-    b = run_gpu(view, wl.camera(), wl.settings(), wl.integrator(), configs.FIELD_WGSL[field], v, f)
-    assert np.array_equal(a.hit_faces, b.hit_faces)
-    assert np.array_equal(a.steps, b.steps)
-    d = np.abs(a.mapping.astype(np.float64) - b.mapping) / np.maximum(1.0, np.abs(a.mapping))
-    assert d.max() <= 1e-5, d.max()
+    try:
+        for math in (L.MATH_IEEE, L.MATH_WGSL):  # the body is compiled for the same build as the functor
+            view.set_math(math)
+            print(f"[case] wgsl-vs-functor {field} math={math}", flush=True)
+            a = run_gpu(view, wl.camera(), wl.settings(), wl.integrator(), field, v, f)
+            b = run_gpu(view, wl.camera(), wl.settings(), wl.integrator(), configs.FIELD_WGSL[field], v, f)
+            assert np.array_equal(a.hit_faces, b.hit_faces)
+            assert np.array_equal(a.steps, b.steps)
+            d = np.abs(a.mapping.astype(np.float64) - b.mapping) / np.maximum(1.0, np.abs(a.mapping))
+            assert d.max() <= 1e-5, d.max()
+    finally:
+        view.set_math(L.MATH_AUTO)
 
 
 def test_wgsl_compile_error_keeps_previous_field(view):
