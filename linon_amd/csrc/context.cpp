@@ -213,11 +213,11 @@ bool Context::wgsl_math_for(int field) const {
     if (math == LINON_MATH_IEEE) return false;
     if (math == LINON_MATH_WGSL) return true;
     switch (field) {  // AUTO
-        case LINON_FIELD_TRANSLATION_X: case LINON_FIELD_TRANSLATION_Z: case LINON_FIELD_ROTATION:
-        case LINON_FIELD_LORENZ: case LINON_FIELD_ROESSLER: case LINON_FIELD_IDENTITY:
-        case LINON_FIELD_SWIRL: case LINON_FIELD_LENS: case LINON_FIELD_WARP:
+        case LINON_FIELD_TRANSLATION_X: case LINON_FIELD_TRANSLATION_Z: case LINON_FIELD_LORENZ:
+        case LINON_FIELD_ROESSLER: case LINON_FIELD_IDENTITY: case LINON_FIELD_SWIRL: case LINON_FIELD_LENS:
+        case LINON_FIELD_WARP:
             return true;
-        default:  // mirages, hot air: refraction
+        default:  // mirages and hot air (refraction), rotation (sin/cos of pi t up to 5 pi)
             return false;
     }
 }

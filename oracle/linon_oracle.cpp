@@ -39,16 +39,15 @@ template <typename T> inline V3<T> operator-(V3<T> a, V3<T> b) { return mk(a.x -
 template <typename T> inline V3<T> operator-(V3<T> a) { return mk(-a.x, -a.y, -a.z); }
 template <typename T> inline V3<T> operator*(T s, V3<T> a) { return mk(s * a.x, s * a.y, s * a.z); }
 template <typename T> inline V3<T> operator*(V3<T> a, T s) { return mk(a.x * s, a.y * s, a.z * s); }
-template <typename T> inline V3<T> operator/(V3<T> a, T s) { return mk(a.x / s, a.y / s, a.z / s); }
 template <typename T> inline T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 template <typename T> inline V3<T> cross(V3<T> a, V3<T> b) {
     return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-// Transcendental rounding probe.  WGSL fixes exp/sin/cos/sqrt only to a few ulp, so two
-// conforming FP32 evaluations of one ray may differ in their last bits.  With g_nudge = +1 or
-// -1 the FP32 instantiation moves every exp, sin, cos and sqrt result of the field and step
-// arithmetic one ulp up or down; a pixel whose end point then moves by more than the parity
-// tolerance is rounding-sensitive.  The FP64 instantiation and g_nudge = 0 are untouched.
+// Rounding probe.  WGSL fixes exp/sin/cos/sqrt and division only to a few ulp, so two conforming
+// FP32 evaluations of one ray may differ in their last bits.  With g_nudge = +1 or -1 the FP32
+// instantiation moves every exp, sin, cos, sqrt and division result of the camera, field and
+// step arithmetic one ulp up or down; a pixel whose end point then moves by more than the
+// parity tolerance is rounding-sensitive.  The FP64 instantiation and g_nudge = 0 are untouched.
 int g_nudge = 0;
 template <typename T> inline T nudged(T r) { return r; }
 template <> inline float nudged<float>(float r) {
@@ -58,6 +57,11 @@ template <typename T> inline T t_sqrt(T x) { return nudged(T(std::sqrt(x))); }
 template <typename T> inline T t_exp(T x) { return nudged(T(std::exp(x))); }
 template <typename T> inline T t_sin(T x) { return nudged(T(std::sin(x))); }
 template <typename T> inline T t_cos(T x) { return nudged(T(std::cos(x))); }
+// division, which WGSL fixes only to 2.5 ulp, is probed the same way
+template <typename T> inline T t_div(T a, T b) { return nudged(T(a / b)); }
+template <typename T> inline V3<T> operator/(V3<T> a, T s) { return mk(t_div(a.x, s), t_div(a.y, s), t_div(a.z, s)); }
+// the RK4 average's division by 6 (the kernels compute it correctly rounded in every build)
+template <typename T> inline V3<T> div_exact(V3<T> a, T s) { return mk(a.x / s, a.y / s, a.z / s); }
 template <typename T> inline T length(V3<T> a) { return t_sqrt(dot(a, a)); }
 template <typename T> inline V3<T> normalize(V3<T> a) { return a / length(a); }
 template <typename T> inline V3<T> vabs(V3<T> a) { return mk(std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)); }
@@ -94,8 +98,8 @@ template <typename T> T refraction_index(T t) {
     const T air_pressure = T(101325.0);
     const T c1 = T(0.0000104);
     const T c2 = T(0.00366);
-    return c1 * air_pressure * (T(1) + air_pressure * (T(60.1) - T(0.972) * t) * T(1e-10)) /
-           (T(1) + c2 * t);
+    return t_div(c1 * air_pressure * (T(1) + air_pressure * (T(60.1) - T(0.972) * t) * T(1e-10)),
+                 T(1) + c2 * t);
 }
 template <typename T> V3<T> refraction(T t_in, T t_out, V3<T> v_in, V3<T> n) {  // :119-136
     T eta_in = refraction_index(t_in);
@@ -107,7 +111,7 @@ template <typename T> V3<T> refraction(T t_in, T t_out, V3<T> v_in, V3<T> n) {  
     } else {
         n_ref = -n;
     }
-    T eta = eta_in / eta_out;
+    T eta = t_div(eta_in, eta_out);
     T k = T(1) - eta * eta * (T(1) - cosi * cosi);
     if (k < T(0)) return reflect(v_in, n_ref);
     return eta * v_in + (eta * cosi - t_sqrt(k)) * n_ref;
@@ -116,7 +120,7 @@ template <typename T> T point_plane_distance(V3<T> p, V3<T> n, V3<T> p0) {  // :
     T d = dot(p0, n);
     return std::fabs(dot(p, n) - d);
 }
-template <typename T> T sigmoid(T x) { return T(1) / (T(1) + t_exp(-x)); }  // :144-146
+template <typename T> T sigmoid(T x) { return t_div(T(1), T(1) + t_exp(-x)); }  // :144-146
 
 // ---------------------------------------------------------------------------------------
 // Field bodies.  Ids 1..9 follow the enum order of src/functions.rs:2-13 (0 = Custom has no
@@ -146,11 +150,11 @@ V3<T> mirage_temperatures(int id, V3<T> p_prev, V3<T> p, T* t_in, T* t_out, V3<T
     }
     T part_in, part_out;
     if (sig) {
-        part_in = wgsl_clamp(T(0), T(1), sigmoid(dist_in / max_dist * T(12) - T(6)));
-        part_out = wgsl_clamp(T(0), T(1), sigmoid(dist_out / max_dist * T(12) - T(6)));
+        part_in = wgsl_clamp(T(0), T(1), sigmoid(t_div(dist_in, max_dist) * T(12) - T(6)));
+        part_out = wgsl_clamp(T(0), T(1), sigmoid(t_div(dist_out, max_dist) * T(12) - T(6)));
     } else {
-        part_in = wgsl_clamp(T(0), T(1), dist_in / max_dist);
-        part_out = wgsl_clamp(T(0), T(1), dist_out / max_dist);
+        part_in = wgsl_clamp(T(0), T(1), t_div(dist_in, max_dist));
+        part_out = wgsl_clamp(T(0), T(1), t_div(dist_out, max_dist));
     }
     *t_in = part_in * t_env + (T(1) - part_in) * t_src;
     *t_out = part_out * t_env + (T(1) - part_out) * t_src;
@@ -227,7 +231,7 @@ T hit_triangle(V3<T> v0, V3<T> v1, V3<T> v2, V3<T> origin, V3<T> direction) {  /
     V3<T> h = cross(direction, edge2);
     T a = dot(edge1, h);
     if (a > -eps && a < eps) return T(-1);  // backface_culling = false (:65)
-    T f = T(1) / a;
+    T f = t_div(T(1), a);
     V3<T> s = origin - v0;
     T u = f * dot(s, h);
     if (u < T(0) || u > T(1)) return T(-1);
@@ -353,8 +357,8 @@ CameraBasis<T> camera_basis(const CameraIn& c, int W, int H) {
 template <typename T>
 V3<T> camera_ray(const CameraBasis<T>& b, int W, int H, int gx, int gy) {
     T width = T(W), height = T(H);
-    T u = T(gx) / (width - T(1)) * b.vw - T(0.5) * b.vw;
-    T v = T(gy) / (height - T(1)) * b.vh - T(0.5) * b.vh;
+    T u = t_div(T(gx), width - T(1)) * b.vw - T(0.5) * b.vw;
+    T v = t_div(T(gy), height - T(1)) * b.vh - T(0.5) * b.vh;
     V3<T> s = u * b.nh + v * b.nv + b.focal * b.view_direction;
     return normalize(s);
 }
@@ -398,7 +402,7 @@ RayOut<T> trace(const Scene<T>& sc, int field, const IntegratorIn& ig, T field_w
         V3<T> k2 = field_eval(field, cur_point, cur_point + (half * h) * k1, start_dir, k1, t + half * h);
         V3<T> k3 = field_eval(field, cur_point, cur_point + (half * h) * k2, start_dir, k2, t + half * h);
         V3<T> k4 = field_eval(field, cur_point, cur_point + h * k3, start_dir, k3, t + h);
-        V3<T> v = (((k1 + T(2) * k2) + T(2) * k3) + k4) / T(6);
+        V3<T> v = div_exact(((k1 + T(2) * k2) + T(2) * k3) + k4, T(6));
         T diff = length(v - last_v);
         if (ig.adaptive) {
             // Conditioning of the adaptive rule (main_view.wgsl:252,267): where the field is
@@ -657,8 +661,8 @@ int sample_rays(const CameraIn* cam, const SettingsIn* st, const IntegratorIn* i
         T py = T(st->mouse_pos[1]) + T(0.01) * T(kSampleOffsets[r][1]);
         if (outline) {  // main_view.wgsl:466-467
             const T width = T(W), height = T(H);
-            px = T(outline[3 * r]) / (width - T(1));
-            py = (height - T(1) - T(outline[3 * r + 1])) / (height - T(1));
+            px = t_div(T(outline[3 * r]), width - T(1));
+            py = t_div(height - T(1) - T(outline[3 * r + 1]), height - T(1));
         }
         T u2 = px * cb.vw - T(0.5) * cb.vw;
         T v2 = py * cb.vh - T(0.5) * cb.vh;
@@ -671,7 +675,7 @@ int sample_rays(const CameraIn* cam, const SettingsIn* st, const IntegratorIn* i
             V3<T> k2 = field_eval(field, cur_point, cur_point + (T(0.5) * h) * k1, start_dir, k1, t + T(0.5) * h);
             V3<T> k3 = field_eval(field, cur_point, cur_point + (T(0.5) * h) * k2, start_dir, k2, t + T(0.5) * h);
             V3<T> k4 = field_eval(field, cur_point, cur_point + h * k3, start_dir, k3, t + h);
-            V3<T> v = (((k1 + T(2) * k2) + T(2) * k3) + k4) / T(6);
+            V3<T> v = div_exact(((k1 + T(2) * k2) + T(2) * k3) + k4, T(6));
             cur_dir = (T(1) - w) * cur_dir + w * v;
             V3<T> step_dir = cur_dir * h;
             if (i % stride == 0 && i / stride < nsamp) {

@@ -133,8 +133,8 @@ def set_prefilter(on: bool) -> None:
 
 
 # FP32 variants of one restatement: "f32" (one rounding per WGSL operation, the parity
-# oracle), "f32fma" (FMA contraction allowed), "f32up" / "f32dn" (every exp, sin, cos and sqrt
-# of the field and step arithmetic moved one ulp up / down).  Their spread around the "f64"
+# oracle), "f32fma" (FMA contraction allowed), "f32up" / "f32dn" (every exp, sin, cos, sqrt and
+# division of the camera, field and step arithmetic moved one ulp up / down).  Their spread around the "f64"
 # result measures how far FP32 rounding alone moves a ray.
 FP32_VARIANTS = ("f32", "f32fma", "f32up", "f32dn")
 

@@ -47,7 +47,7 @@ def run_oracle(cam, st, ig, fid, verts, faces, W, H, precision="f32"):
 
 
 # FP32 variants besides the parity oracle itself whose end points measure a pixel's rounding
-# sensitivity: FMA contraction allowed, and exp/sin/cos/sqrt moved one ulp up or down.
+# sensitivity: FMA contraction allowed, and exp/sin/cos/sqrt/division moved one ulp up or down.
 PROBE_VARIANTS = ("f32fma", "f32up", "f32dn")
 
 
@@ -92,7 +92,7 @@ def low_lambda_mask(mapping, delta=1):
 
 
 # A pixel is well conditioned when every FP32 variant of the restatement (the oracle, FMA
-# contraction allowed, transcendentals one ulp up, one ulp down) ends within WELL_COND of the
+# contraction allowed, transcendentals and divisions one ulp up, one ulp down) ends within WELL_COND of the
 # FP64 end point; there the GPU must match the FP32 oracle to EXIT_RTOL.  Elsewhere FP32
 # rounding alone moves the end point by more than the tolerance (e.g. the cancellation in
 # refraction's `eta * cosi - sqrt(k)` near grazing incidence, or a total-internal-reflection
