@@ -193,7 +193,7 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl.name, "width": wl.width, "height": wl.height, "fields": list(wl.fields),
-                   "math": "auto: WGSL accuracy for swirl/lens/warp, IEEE for hot-air",
+                   "math": "auto: WGSL accuracy for swirl/warp, IEEE for lens/hot-air",
                    "h": wl.h, "t_max": wl.t_max, "max_steps": wl.max_steps, "adaptive": wl.adaptive,
                    "overlay": wl.overlay, "faces": int(len(faces)), "sample_rays": wl.samples,
                    "step": "one 1080p frame per field (4 frames)"},

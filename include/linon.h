@@ -234,11 +234,11 @@ linon_status linon_read_grid(linon_context* ctx, int32_t* dims, uint32_t* cell_s
  * [-pi, pi], exp 3 + 2|x| ulp).  IEEE: correctly rounded division and sqrt, full-precision
  * transcendentals.  WGSL: hardware division, sqrt and transcendentals within those bounds.
  * AUTO (default): WGSL for the predefined fields whose end points and hit faces measure as
- * accurate under it (translations, Lorenz, Roessler, identity, swirl, lens, warp); IEEE for the
+ * accurate under it (translations, Lorenz, Roessler, identity, swirl, warp); IEEE for the
  * refraction fields (the four mirages, hot air), where the eta*cos - sqrt(k) cancellation near
  * grazing incidence amplifies division rounding, for rotation (hardware sin/cos of pi*t up to
- * 5 pi moves hit faces), and for WGSL bodies.  (No counterpart in the reference; its driver
- * decides.) */
+ * 5 pi moves hit faces), for lens (rays captured near the r^3 singularity end elsewhere), and
+ * for WGSL bodies.  (No counterpart in the reference; its driver decides.) */
 typedef enum linon_math {
     LINON_MATH_AUTO = 0,
     LINON_MATH_IEEE = 1,

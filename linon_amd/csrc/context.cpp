@@ -214,11 +214,10 @@ bool Context::wgsl_math_for(int field) const {
     if (math == LINON_MATH_WGSL) return true;
     switch (field) {  // AUTO
         case LINON_FIELD_TRANSLATION_X: case LINON_FIELD_TRANSLATION_Z: case LINON_FIELD_LORENZ:
-        case LINON_FIELD_ROESSLER: case LINON_FIELD_IDENTITY: case LINON_FIELD_SWIRL: case LINON_FIELD_LENS:
-        case LINON_FIELD_WARP:
+        case LINON_FIELD_ROESSLER: case LINON_FIELD_IDENTITY: case LINON_FIELD_SWIRL: case LINON_FIELD_WARP:
             return true;
-        default:  // mirages and hot air (refraction), rotation (sin/cos of pi t up to 5 pi)
-            return false;
+        default:  // mirages and hot air (refraction), rotation (sin/cos of pi t up to 5 pi),
+            return false;  // lens (captured rays near the r^3 singularity)
     }
 }
 

@@ -59,9 +59,9 @@ def test_config_camera_cornell(view, field):
 @pytest.mark.parametrize("math", ["ieee", "wgsl"])
 @pytest.mark.parametrize("field", ["swirl", "lens", "warp"])
 def test_math_builds(view, field, math):
-    """Both kernel builds against the oracle: IEEE (correctly rounded division and sqrt,
-    full-precision transcendentals) and WGSL accuracy (hardware division, sqrt, sin/cos/exp
-    within WGSL's bounds), which AUTO selects for these fields."""
+    """Both kernel builds against the oracle on the 64x64 Cornell view: IEEE (correctly rounded
+    division and sqrt, full-precision transcendentals) and WGSL accuracy (hardware division,
+    sqrt, sin/cos/exp within WGSL's bounds; AUTO's choice for swirl and warp)."""
     wl = configs.CONFIG1
     v, f = wl.mesh()
     view.set_math({"ieee": L.MATH_IEEE, "wgsl": L.MATH_WGSL}[math])
