@@ -324,7 +324,10 @@ def bench_surrogate(steps: int, fields) -> dict:
                       "conf_fallback": int(st["conf_fallback"]), "lyapunov_fallback": int(st["lyapunov_fallback"]),
                       "rejudged": int(st["rejudged"]), "rejudge_err_q50": st["rejudge_err_q50"],
                       "rejudge_err_q99": st["rejudge_err_q99"],
-                      "phases_ms": {k: st[k] for k in ("approach_ms", "mlp_ms", "fallback_ms", "overlay_ms")}}
+                      "phases_ms": {k: st[k] for k in ("approach_ms", "mlp_ms", "fallback_ms", "overlay_ms")},
+                      # algorithmic MLP FLOP per ray: 2 x (198*256 + 3*256*256 + 256*7) = 498,176
+                      "mlp_tflops": (int(st["entered"]) * 498176 / (st["mlp_ms"] * 1e-3) / 1e12
+                                     if st["mlp_ms"] > 0 else None)}
         view.close()
         log(f"surrogate {field}: {out[field]['speedup']}x, within 1e-3: {out[field]['frac_pixels_within_1e-3']:.4f}")
     return out

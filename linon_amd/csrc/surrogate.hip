@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device/fields.hpp"
+#include "device/gemm_nt.hpp"
 #include "device/mlp.hpp"
 #include "device/surrogate.hpp"
 #include "surrogate_kernels.hpp"
@@ -13,6 +14,7 @@ namespace LINON_KERNEL_NS {
 using lm::u16;
 
 __global__ void __launch_bounds__(256) gemm_kernel(lm::GemmParams P) { lm::gemm_body(P); }
+__global__ void __launch_bounds__(256) gemm_nt_kernel(lm::GemmParams P) { lm::gemm_nt_body(P); }
 
 __global__ void __launch_bounds__(256) features_kernel(ls::SurrogateIO S) { ls::features_body(S); }
 
@@ -299,8 +301,12 @@ __global__ void __launch_bounds__(256) fill_i32_kernel(int* p, int n, int value)
 
 // ---- launchers ------------------------------------------------------------------------------
 hipError_t sg_gemm(const lm::GemmParams& P, int splits, hipStream_t s) {
-    dim3 grid((P.M + LM_BM - 1) / LM_BM, (P.N + LM_BN - 1) / LM_BN, splits);
     void* args[] = {const_cast<lm::GemmParams*>(&P)};
+    if (splits == 1 && lm::gemm_nt_applies(P)) {  // the forward layout: 128 x 128 tiles, 16-B loads
+        dim3 grid((P.M + LM_NT_BM - 1) / LM_NT_BM, P.N / LM_NT_BN, 1);
+        return hipLaunchKernel((const void*)&gemm_nt_kernel, grid, dim3(256), args, 0, s);
+    }
+    dim3 grid((P.M + LM_BM - 1) / LM_BM, (P.N + LM_BN - 1) / LM_BN, splits);
     return hipLaunchKernel((const void*)&gemm_kernel, grid, dim3(256), args, 0, s);
 }
 

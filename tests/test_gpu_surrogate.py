@@ -42,6 +42,8 @@ def run_gemm(A16, B16, M, N, K, a_trans, b_trans, splits=1, bias=None, silu=Fals
     (200, 96, 224, 0, 1, 1),     # ragged M/N, forward layout (A [M][K], B stored [N][K])
     (130, 256, 256, 0, 0, 1),    # backward dX layout (B stored [K][N])
     (224, 256, 4096, 1, 0, 4),   # weight-gradient layout (A stored [K][M]) with split-K
+    (1000, 256, 224, 0, 1, 1),   # forward layout through the 128x128 fast path (ragged M, K = 224)
+    (4096, 256, 256, 0, 1, 1),   # fast path, full tiles
 ])
 def test_mfma_gemm_matches_numpy(M, N, K, a_trans, b_trans, splits):
     rng = np.random.default_rng(M * 7 + N + K)
@@ -55,13 +57,15 @@ def test_mfma_gemm_matches_numpy(M, N, K, a_trans, b_trans, splits):
     np.testing.assert_allclose(got, ref, rtol=1e-4, atol=1e-4 * np.sqrt(K))
 
 
-def test_gemm_bias_silu_epilogue():
+@pytest.mark.parametrize("M,N,K", [(96, 64, 64), (300, 256, 224)])   # generic path, fast path
+def test_gemm_bias_silu_epilogue(M, N, K):
     rng = np.random.default_rng(5)
-    M, N, K = 96, 64, 64
     A, B = rng.normal(size=(M, K)).astype(np.float32), rng.normal(size=(K, N)).astype(np.float32)
     bias = rng.normal(size=N).astype(np.float32)
     z = bf16_value(bf16_bits(A)).astype(np.float64) @ bf16_value(bf16_bits(B)).astype(np.float64) + bias
-    got_z, got16 = run_gemm(bf16_bits(A), bf16_bits(B), M, N, K, 0, 0, 1, bias, silu=True)
+    b_trans = 1 if N % 128 == 0 else 0   # the fast path needs B stored [N][K]
+    B16 = bf16_bits(B.T.copy()) if b_trans else bf16_bits(B)
+    got_z, got16 = run_gemm(bf16_bits(A), B16, M, N, K, 0, b_trans, 1, bias, silu=True)
     np.testing.assert_allclose(got_z, z, rtol=1e-4, atol=1e-3)
     silu = z / (1 + np.exp(-z))
     np.testing.assert_allclose(bf16_value(got16), silu, rtol=1e-2, atol=1e-2)
