@@ -276,12 +276,26 @@ def bench_batch(verts, faces, replays: int) -> dict:
     ms, _ = view.timings()
     dev_ms = float(np.mean(ms))
     rays = sum(v.width * v.height for v in views)
+    # the same views one after another through the plain kernel (escape mode, same steps)
+    plain_ms = 0.0
+    ig = configs.batch_integrator(max_steps=8000)
+    for bv in views:
+        view.resize_texture(bv.width, bv.height)
+        view.update_camera(bv.camera)
+        view.update_settings(L.Settings.make(field_weight=bv.field_weight))
+        ig.h_initial = bv.h_initial
+        view.set_integrator(ig)
+        view.set_field(bv.field)
+        view.reset_timings()
+        view.render()
+        plain_ms += float(view.timings()[0][0])
     view.close()
     return {"workload": "65 views (seed 0): resolutions {256^2..2048^2}, eye radius U(2.5,4), fields swirl/lens/"
                         "hot-air/warp, h in {0.01,0.005,0.0025} + one Enhance view at h/4; escape mode, max 8000",
             "views": len(views), "rays": rays, "rk4_steps": c["rk4_steps"], "device_ms": dev_ms,
             "wall_ms": wall * 1e3, "gsteps_per_s": c["rk4_steps"] / (dev_ms * 1e-3) / 1e9,
-            "mrays_per_s": rays / (dev_ms * 1e-3) / 1e6, "refills": None,
+            "mrays_per_s": rays / (dev_ms * 1e-3) / 1e6, "plain_kernel_sequential_ms": plain_ms,
+            "speedup_vs_plain_sequential": plain_ms / dev_ms if dev_ms > 0 else None,
             "tuning": tuning.as_dict(), "autotune_proxy_ms": proxy_ms, "autotune_seconds": tune_s}
 
 
