@@ -33,22 +33,55 @@ __device__ __forceinline__ vec3 ld_tri_vec(const float* tris, int face, int k) {
     return vec3f(q.x, q.y, q.z);
 }
 
+// WGSL-literal arithmetic: every operation rounded once, no FMA contraction, in the order the
+// shader text writes it.  The integrator, camera and sample-ray statements use it, so every
+// kernel that advances a ray (plain, batched, the surrogate's RK4 legs) rounds them alike
+// whatever the surrounding code lets the compiler fuse, and as the FP32 oracle does.
+namespace rn {
+__device__ __forceinline__ float mul(float a, float b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ float add(float a, float b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ float sub(float a, float b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
+__device__ __forceinline__ vec3 mul(float s, vec3 a) { return vec3f(mul(s, a.x), mul(s, a.y), mul(s, a.z)); }
+__device__ __forceinline__ vec3 mul(vec3 a, float s) { return vec3f(mul(a.x, s), mul(a.y, s), mul(a.z, s)); }
+__device__ __forceinline__ vec3 add(vec3 a, vec3 b) { return vec3f(add(a.x, b.x), add(a.y, b.y), add(a.z, b.z)); }
+__device__ __forceinline__ vec3 sub(vec3 a, vec3 b) { return vec3f(sub(a.x, b.x), sub(a.y, b.y), sub(a.z, b.z)); }
+__device__ __forceinline__ float dot(vec3 a, vec3 b) { return add(add(mul(a.x, b.x), mul(a.y, b.y)), mul(a.z, b.z)); }
+__device__ __forceinline__ vec3 cross(vec3 a, vec3 b) {
+    return vec3f(sub(mul(a.y, b.z), mul(a.z, b.y)), sub(mul(a.z, b.x), mul(a.x, b.z)),
+                 sub(mul(a.x, b.y), mul(a.y, b.x)));
+}
+__device__ __forceinline__ float length(vec3 a) { return __builtin_sqrtf(rn::dot(a, a)); }
+__device__ __forceinline__ vec3 normalize(vec3 a) {
+    const float l = rn::length(a);
+    return vec3f(a.x / l, a.y / l, a.z / l);
+}
+}  // namespace rn
+
 // Moeller-Trumbore in the operation order of hit_triangle (src/main_view.wgsl:150-175):
 // two-sided, |a| < 1e-7 rejected, inclusive edges, t > 1e-7 accepted.  e1 = v1 - v0 and
 // e2 = v2 - v0 are precomputed in f32, which gives the same bits as computing them here.
 __device__ __forceinline__ float mt_test(vec3 v0, vec3 e1, vec3 e2, vec3 o, vec3 d) {
     const float eps = 0.0000001f;
-    vec3 h = cross(d, e2);
-    float a = dot(e1, h);
+    vec3 h = rn::cross(d, e2);
+    float a = rn::dot(e1, h);
     if (a > -eps && a < eps) return -1.0f;
     float f = 1.0f / a;
-    vec3 s = o - v0;
-    float u = f * dot(s, h);
+    vec3 s = rn::sub(o, v0);
+    float u = rn::mul(f, rn::dot(s, h));
     if (u < 0.0f || u > 1.0f) return -1.0f;
-    vec3 q = cross(s, e1);
-    float v = f * dot(d, q);
-    if (v < 0.0f || u + v > 1.0f) return -1.0f;
-    float t = f * dot(e2, q);
+    vec3 q = rn::cross(s, e1);
+    float v = rn::mul(f, rn::dot(d, q));
+    if (v < 0.0f || rn::add(u, v) > 1.0f) return -1.0f;
+    float t = rn::mul(f, rn::dot(e2, q));
     return t > eps ? t : -1.0f;
 }
 
@@ -199,38 +232,6 @@ __device__ __forceinline__ unsigned pack_rgba8(float r, float g, float b, float 
     return unorm8(r) | (unorm8(g) << 8) | (unorm8(b) << 16) | (unorm8(a) << 24);
 }
 
-// WGSL-literal arithmetic: every operation rounded once, no FMA contraction, in the order the
-// shader text writes it.  The integrator, camera and sample-ray statements use it, so every
-// kernel that advances a ray (plain, batched, the surrogate's RK4 legs) rounds them alike
-// whatever the surrounding code lets the compiler fuse, and as the FP32 oracle does.
-namespace rn {
-__device__ __forceinline__ float mul(float a, float b) {
-#pragma clang fp contract(off)
-    return a * b;
-}
-__device__ __forceinline__ float add(float a, float b) {
-#pragma clang fp contract(off)
-    return a + b;
-}
-__device__ __forceinline__ float sub(float a, float b) {
-#pragma clang fp contract(off)
-    return a - b;
-}
-__device__ __forceinline__ vec3 mul(float s, vec3 a) { return vec3f(mul(s, a.x), mul(s, a.y), mul(s, a.z)); }
-__device__ __forceinline__ vec3 mul(vec3 a, float s) { return vec3f(mul(a.x, s), mul(a.y, s), mul(a.z, s)); }
-__device__ __forceinline__ vec3 add(vec3 a, vec3 b) { return vec3f(add(a.x, b.x), add(a.y, b.y), add(a.z, b.z)); }
-__device__ __forceinline__ vec3 sub(vec3 a, vec3 b) { return vec3f(sub(a.x, b.x), sub(a.y, b.y), sub(a.z, b.z)); }
-__device__ __forceinline__ float dot(vec3 a, vec3 b) { return add(add(mul(a.x, b.x), mul(a.y, b.y)), mul(a.z, b.z)); }
-__device__ __forceinline__ vec3 cross(vec3 a, vec3 b) {
-    return vec3f(sub(mul(a.y, b.z), mul(a.z, b.y)), sub(mul(a.z, b.x), mul(a.x, b.z)),
-                 sub(mul(a.x, b.y), mul(a.y, b.x)));
-}
-__device__ __forceinline__ float length(vec3 a) { return __builtin_sqrtf(rn::dot(a, a)); }
-__device__ __forceinline__ vec3 normalize(vec3 a) {
-    const float l = rn::length(a);
-    return vec3f(a.x / l, a.y / l, a.z / l);
-}
-}  // namespace rn
 
 // Camera ray of invocation (gx, gy), src/main_view.wgsl:347-366.
 struct CameraFrame {
