@@ -22,11 +22,11 @@ def setup():
     view.close()
 
 
-def _plain(view, bv):
+def _plain(view, bv, max_steps=3000):
     view.resize_texture(bv.width, bv.height)
     view.update_camera(bv.camera)
     view.update_settings(L.Settings.make(field_weight=bv.field_weight))
-    ig = configs.batch_integrator(max_steps=3000)
+    ig = configs.batch_integrator(max_steps=max_steps)
     ig.h_initial = bv.h_initial
     view.set_integrator(ig)
     view.set_field(bv.field)
@@ -106,3 +106,29 @@ def test_batch_argument_errors(setup):
         view.set_batch(bad)
     with pytest.raises(L.LinonError):
         view.set_batch_tuning(0, 1, 1, 1)
+
+
+def test_full_resolution_batch_equals_plain():
+    """Size-independent property at the benchmark's own sizes: four 1920x1080 views, one per
+    field, different eyes and step sizes, in one batch equal the plain kernel's frames bit for
+    bit."""
+    v, f = scenes.benchmark_scene(level=5)
+    view = L.MainView(1920, 1080)
+    try:
+        view.update_model(v, f)
+        view.set_integrator(configs.batch_integrator(max_steps=4000))
+        eyes = [(0.0, 0.0, 3.0), (2.0, 1.0, 2.2), (-2.5, 0.5, 1.5), (0.3, -2.8, 1.0)]
+        steps = [0.005, 0.01, 0.0025, 0.005]
+        views = [L.View.make(L.Camera.look_at(e, (0, 0, 0), fov_deg=60.0), 1920, 1080, name, h)
+                 for e, name, h in zip(eyes, ("swirl", "lens", "hot_air", "warp"), steps)]
+        print("[case] full-resolution batch of 4 views", flush=True)
+        view.set_batch(views)
+        view.render_batch()
+        got = [view.read_batch_view(i) for i in range(len(views))]
+        for i, bv in enumerate(views):
+            ref = _plain(view, bv, max_steps=4000)
+            assert np.array_equal(got[i].hit_faces, ref.hit_faces), i
+            assert np.array_equal(got[i].mapping, ref.mapping), i
+            assert np.array_equal(got[i].steps, ref.steps), i
+    finally:
+        view.close()
