@@ -1,5 +1,6 @@
 """Writes oracle_fixtures.npz: small golden cases of the ray-cast pass, inputs and outputs of
-the CPU oracle (FP32 restatement, its FMA-contracted build and the FP64 instantiation).
+the CPU oracle (the FP32 restatement, its rounding probes: the FMA-contracted build and the
+builds with transcendentals and divisions one ulp up / down, and the FP64 instantiation).
 
 Each case stores its inputs in full (camera / settings / integrator structs as raw bytes, the
 mesh arrays, field id, image size), so the GPU tests replay exactly these inputs, and the
@@ -55,7 +56,8 @@ def main() -> None:
                    O.as_oracle_struct(ig, O.IntegratorIn))
         r32 = O.render(c, s, i, fid, v, f, W, H, threads=8)
         r64 = O.render(c, s, i, fid, v, f, W, H, threads=8, precision="f64")
-        rf = O.render(c, s, i, fid, v, f, W, H, threads=8, precision="f32fma")
+        probes = {var: O.render(c, s, i, fid, v, f, W, H, threads=8, precision=var)
+                  for var in ("f32fma", "f32up", "f32dn")}
         ov = O.overlay(O.settings_in(overlay_mode=1, delta=1), r32["color8"], r32["mapping"])
         names.append(name)
         p = name + "/"
@@ -68,7 +70,9 @@ def main() -> None:
         data[p + "steps"] = r32["steps"]
         data[p + "color8"] = r32["color8"]
         data[p + "mapping_f32"] = r32["mapping"]
-        data[p + "mapping_f32fma"] = rf["mapping"]
+        for var, rv in probes.items():
+            data[p + "mapping_" + var] = rv["mapping"]
+            data[p + "hit_" + var] = rv["hit"]
         data[p + "mapping_f64"] = r64["mapping"]
         data[p + "lyapunov_f32"] = ov["lyapunov"]
         data[p + "noise_flag"] = r32["noise_flag"]

@@ -36,7 +36,9 @@ def test_gpu_matches_golden_case(view, name):
     p = name + "/"
     ref = {"mapping": Z[p + "mapping_f32"], "hit": Z[p + "hit"], "noise_flag": Z[p + "noise_flag"],
            "color8": Z[p + "color8"]}
-    ref64 = {"mapping": Z[p + "mapping_f64"], "mapping_f32fma": Z[p + "mapping_f32fma"]}
+    ref64 = {"mapping": Z[p + "mapping_f64"]}
+    for var in ("f32fma", "f32up", "f32dn"):
+        ref64["mapping_" + var], ref64["hit_" + var] = Z[p + "mapping_" + var], Z[p + "hit_" + var]
     info = compare(g, ref, ref64, f"golden-{name}")
     if info["gated"] == 0:
         pytest.skip("no low-lambda pixels in this case")
