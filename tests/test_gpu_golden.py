@@ -40,8 +40,7 @@ def test_gpu_matches_golden_case(view, name):
     for var in ("f32fma", "f32up", "f32dn"):
         ref64["mapping_" + var], ref64["hit_" + var] = Z[p + "mapping_" + var], Z[p + "hit_" + var]
     info = compare(g, ref, ref64, f"golden-{name}")
-    if info["gated"] == 0:
-        pytest.skip("no low-lambda pixels in this case")
+    assert info["gated"] > 0, info
     assert info["hit_ok"], info
     assert info["exit_pass"], info
     assert info["q99_ok"], info
