@@ -301,6 +301,13 @@ linon_status linon_surrogate_train(linon_context* ctx, const linon_surrogate_con
 linon_status linon_render_surrogate(linon_context* ctx, void* stream);
 linon_status linon_get_surrogate_stats(linon_context* ctx, linon_surrogate_stats* out);
 
+/* Diagnostics: the shared-denominator division of the IEEE build (lw::div3_rn) against the
+ * IEEE division, bit for bit: the reciprocal for every mantissa at exponents -59..59 (both
+ * signs), every numerator mantissa at three exponents against 16 structured denominators, and
+ * random_pairs random operand pairs (zeros and out-of-range magnitudes included).  Writes the
+ * mismatch counts and the number of comparisons. */
+linon_status linon_selftest_division(int device, uint64_t random_pairs, uint64_t* recip_mismatches,
+                                     uint64_t* quotient_mismatches, uint64_t* comparisons);
 /* Diagnostics: the surrogate's BF16-MFMA GEMM on host buffers (bf16 bit patterns in, fp32
  * out): C = A * B (+ bias; with silu != 0: C = z = A*B + bias and C_bf16 = bf16(silu(z))).
  * A is [M][K] (a_trans = 0) or [K][M]; B is [K][N] (b_trans = 0) or [N][K]; the K range may

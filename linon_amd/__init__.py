@@ -231,6 +231,8 @@ SIGNATURES = {
     "linon_set_accel": (C.c_int, [_P, C.c_int, C.c_int]),
     "linon_set_grid_builder": (C.c_int, [_P, C.c_int]),
     "linon_set_math": (C.c_int, [_P, C.c_int]),
+    "linon_selftest_division": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                          C.POINTER(C.c_uint64)]),
     "linon_read_grid": (C.c_int, [_P, _P, _P, C.c_size_t, _P, C.c_size_t, _P, C.c_size_t, _P, C.c_size_t, _P]),
     "linon_set_field_predefined": (C.c_int, [_P, C.c_int]),
     "linon_set_field_wgsl": (C.c_int, [_P, C.c_char_p]),
@@ -552,3 +554,15 @@ class MainView:
     def read_frame(self) -> Frame:
         return Frame(self.read_final(), self.read_ray_casting(), self.read_mapping(), self.read_hit_faces(),
                      self.read_steps(), self.read_lyapunov(), self.read_exponents(), self.counters())
+
+
+def selftest_division(random_pairs: int = 1 << 28, device: int = 0) -> dict:
+    """Bit-for-bit check of the IEEE build's shared-denominator division against IEEE division
+    (on the GPU): mismatch counts of the reciprocal and of the quotients, and the number of
+    comparisons."""
+    lib = load_library()
+    r, q, n = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+    st = lib.linon_selftest_division(device, random_pairs, C.byref(r), C.byref(q), C.byref(n))
+    if st != LINON_OK:
+        raise LinonError(st, "linon_selftest_division failed")
+    return {"recip_mismatches": r.value, "quotient_mismatches": q.value, "comparisons": n.value}

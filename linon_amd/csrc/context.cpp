@@ -456,6 +456,18 @@ linon_status linon_set_accel(linon_context* ctx, int accel, int grid_resolution)
     return LINON_OK;
 }
 
+linon_status linon_selftest_division(int device, uint64_t random_pairs, uint64_t* recip_mismatches,
+                                     uint64_t* quotient_mismatches, uint64_t* comparisons) {
+    if (!recip_mismatches || !quotient_mismatches || !comparisons) return LINON_ERR_ARGUMENT;
+    if (hipSetDevice(device) != hipSuccess) return LINON_ERR_HIP;
+    unsigned long long r = 0, q = 0, n = 0;
+    if (linon::run_division_selftest(random_pairs, &r, &q, &n) != hipSuccess) return LINON_ERR_HIP;
+    *recip_mismatches = r;
+    *quotient_mismatches = q;
+    *comparisons = n;
+    return LINON_OK;
+}
+
 linon_status linon_set_math(linon_context* ctx, int math) {
     Context* c = reinterpret_cast<Context*>(ctx);
     LN_CTX(c);

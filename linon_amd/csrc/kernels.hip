@@ -108,4 +108,108 @@ hipError_t launch_clear_u64(unsigned long long* p, int n, hipStream_t s) {
     return hipLaunchKernel((const void*)&clear_u64_kernel, dim3((n + 63) / 64), dim3(64), args, 0, s);
 }
 
+// ---- self-test of lw::div3_rn / lw::div_rn against the IEEE division (IEEE build only) ----
+#ifndef LINON_FAST_MATH
+__device__ __forceinline__ unsigned fbits(float x) { return __float_as_uint(x); }
+
+// every mantissa of b at exponent e (both signs): recip_rn(b) against 1.0f / b
+__global__ void __launch_bounds__(256) div_selftest_recip_kernel(int e, unsigned long long* mism) {
+    const unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (1u << 23)) return;
+    unsigned bad = 0;
+    for (int sgn = 0; sgn < 2; ++sgn) {
+        const float b = __uint_as_float(((unsigned)sgn << 31) | ((unsigned)(e + 127) << 23) | m);
+        if (lw::div_rn_range(b) && fbits(lw::recip_rn(b)) != fbits(1.0f / b)) ++bad;
+    }
+    if (bad) atomicAdd(mism, (unsigned long long)bad);
+}
+
+// every mantissa of a at exponent ea against one b: quot through div3_rn against a / b
+__global__ void __launch_bounds__(256) div_selftest_mantissa_kernel(int ea, float b, unsigned long long* mism) {
+    const unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (1u << 23)) return;
+    const float a = __uint_as_float(((unsigned)(ea + 127) << 23) | m);
+    const lw::vec3 q = lw::div3_rn(lw::vec3f(a, -a, 0.75f * a), b);
+    unsigned bad = 0;
+    bad += fbits(q.x) != fbits(a / b);
+    bad += fbits(q.y) != fbits(-a / b);
+    bad += fbits(q.z) != fbits((0.75f * a) / b);
+    if (bad) atomicAdd(mism, (unsigned long long)bad);
+}
+
+// random operands (SplitMix64 stream): sign, exponent in [-70, 70] (so the out-of-range and
+// zero paths are taken too), mantissa uniform
+__device__ __forceinline__ unsigned long long smix(unsigned long long z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ float rand_float(unsigned long long r) {
+    const unsigned m = (unsigned)(r & 0x7fffffu);
+    const int ex = (int)((r >> 23) % 141u) - 70;
+    const unsigned sg = (unsigned)((r >> 40) & 1u);
+    if (((r >> 41) & 1023u) == 0) return sg ? -0.0f : 0.0f;
+    return __uint_as_float((sg << 31) | ((unsigned)(ex + 127) << 23) | m);
+}
+__global__ void __launch_bounds__(256) div_selftest_random_kernel(unsigned long long seed, unsigned long long n,
+                                                                  unsigned long long* mism) {
+    const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long r1 = smix(seed + 0x9E3779B97F4A7C15ull * (2 * i + 1));
+    const unsigned long long r2 = smix(seed + 0x9E3779B97F4A7C15ull * (2 * i + 2));
+    const float a = rand_float(r1), b = rand_float(r2);
+    const float c = rand_float(r1 ^ r2);
+    const lw::vec3 q = lw::div3_rn(lw::vec3f(a, c, -a), b);
+    unsigned bad = 0;
+    bad += fbits(q.x) != fbits(a / b);
+    bad += fbits(q.y) != fbits(c / b);
+    bad += fbits(q.z) != fbits(-a / b);
+    bad += fbits(lw::div_rn(a, b)) != fbits(a / b);
+    if (bad) atomicAdd(mism, (unsigned long long)bad);
+}
+#endif
+
+#ifndef LINON_FAST_MATH
+hipError_t run_division_selftest(unsigned long long random_pairs, unsigned long long* recip_mism,
+                                 unsigned long long* quot_mism, unsigned long long* cases) {
+    unsigned long long* d = nullptr;
+    hipError_t e = hipMalloc(&d, 2 * sizeof(unsigned long long));
+    if (e != hipSuccess) return e;
+    e = hipMemset(d, 0, 2 * sizeof(unsigned long long));
+    unsigned long long n = 0;
+    const unsigned mblocks = (1u << 23) / 256;
+    for (int ex = -59; ex <= 59 && e == hipSuccess; ++ex) {  // every mantissa, both signs
+        hipLaunchKernelGGL(div_selftest_recip_kernel, dim3(mblocks), dim3(256), 0, nullptr, ex, d);
+        n += 2ull << 23;
+    }
+    // every numerator mantissa at three exponents against structured denominators: 1, 1 + ulp,
+    // 2 - ulp (all-ones mantissa), thirds, tenths, and the field constants' neighbourhood
+    const float denoms[] = {1.0f, 1.00000012f, 1.99999988f, 3.0f, 0.1f, 0.001f, 7.77f, 1.41421354f,
+                            0.333333343f, 1e5f, 2.5e-7f, 1.5f, 1.0000024f, 1.9999976f, 12345.678f, 0.0517f};
+    for (float b : denoms)
+        for (int ea : {-20, 0, 21}) {
+            if (e != hipSuccess) break;
+            hipLaunchKernelGGL(div_selftest_mantissa_kernel, dim3(mblocks), dim3(256), 0, nullptr, ea, b, d + 1);
+            n += 3ull << 23;
+        }
+    if (random_pairs) {
+        const unsigned long long chunk = 1ull << 26;
+        for (unsigned long long off = 0; off < random_pairs && e == hipSuccess; off += chunk) {
+            const unsigned long long m = random_pairs - off < chunk ? random_pairs - off : chunk;
+            hipLaunchKernelGGL(div_selftest_random_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, nullptr,
+                               0xD1B54A32D192ED03ull + off, m, d + 1);
+            n += 4 * m;
+        }
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    unsigned long long h[2] = {0, 0};
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    *recip_mism = h[0];
+    *quot_mism = h[1];
+    *cases = n;
+    return e;
+}
+#endif
 }  // namespace LINON_KERNEL_NS

@@ -26,6 +26,10 @@
 
 namespace linon {
 LINON_DECLARE_KERNEL_LAUNCHERS
+// IEEE build only: lw::div3_rn / lw::div_rn against the IEEE division, every mantissa of the
+// reciprocal and of numerators against structured denominators, plus random operand pairs
+hipError_t run_division_selftest(unsigned long long random_pairs, unsigned long long* recip_mism,
+                                 unsigned long long* quot_mism, unsigned long long* cases);
 }  // namespace linon
 namespace linon_wgslmath {
 LINON_DECLARE_KERNEL_LAUNCHERS

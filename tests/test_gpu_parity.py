@@ -306,3 +306,15 @@ def test_wgsl_compile_error_keeps_previous_field(view):
     view.render()
     b = view.read_frame()
     assert np.array_equal(a.mapping, b.mapping)
+
+
+def test_shared_denominator_division_is_ieee():
+    """The IEEE build divides vec3 by a scalar through one shared, correctly rounded reciprocal
+    and Markstein's correction; it must equal IEEE division bit for bit: every mantissa of the
+    reciprocal at exponents -59..59, every numerator mantissa against 16 structured
+    denominators, and 2^28 random pairs (zeros, out-of-range and signed operands included)."""
+    print("[case] division self-test", flush=True)
+    res = L.selftest_division(random_pairs=1 << 28)
+    print(res, flush=True)
+    assert res["comparisons"] > 3_000_000_000
+    assert res["recip_mismatches"] == 0 and res["quotient_mismatches"] == 0, res
