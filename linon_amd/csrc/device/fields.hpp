@@ -31,11 +31,11 @@ struct FieldMirage {  // src/functions.rs:36-96
         }
         float part_in, part_out;
         if (Sigmoid) {
-            part_in = clamp(0.0f, 1.0f, sigmoid(fdiv(dist_in, max_dist) * 12.0f - 6.0f));
-            part_out = clamp(0.0f, 1.0f, sigmoid(fdiv(dist_out, max_dist) * 12.0f - 6.0f));
+            part_in = clamp(0.0f, 1.0f, sigmoid(dist_in / max_dist * 12.0f - 6.0f));
+            part_out = clamp(0.0f, 1.0f, sigmoid(dist_out / max_dist * 12.0f - 6.0f));
         } else {
-            part_in = clamp(0.0f, 1.0f, fdiv(dist_in, max_dist));
-            part_out = clamp(0.0f, 1.0f, fdiv(dist_out, max_dist));
+            part_in = clamp(0.0f, 1.0f, dist_in / max_dist);
+            part_out = clamp(0.0f, 1.0f, dist_out / max_dist);
         }
         float t_in = part_in * t_env + (1.0f - part_in) * t_src;
         float t_out = part_out * t_env + (1.0f - part_out) * t_src;

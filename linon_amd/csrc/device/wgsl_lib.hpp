@@ -61,11 +61,8 @@ __device__ __forceinline__ vec3 div3_rn(vec3 v, float s) {
 }
 #ifdef LINON_FAST_MATH
 __device__ __forceinline__ vec3 operator/(vec3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
-__device__ __forceinline__ float fdiv(float a, float b) { return a / b; }
 #else
 __device__ __forceinline__ vec3 operator/(vec3 a, float s) { return div3_rn(a, s); }
-// a / b, correctly rounded, for the scalar divisions of the helpers and predefined fields
-__device__ __forceinline__ float fdiv(float a, float b) { return div_rn(a, b); }
 #endif
 __device__ __forceinline__ vec3 operator/(float s, vec3 a) { return {s / a.x, s / a.y, s / a.z}; }
 __device__ __forceinline__ vec3 operator+(vec3 a, float s) { return {a.x + s, a.y + s, a.z + s}; }
@@ -193,7 +190,7 @@ __device__ __forceinline__ float refraction_index(float t) {
     const float air_pressure = 101325.0f;
     const float c1 = 0.0000104f;
     const float c2 = 0.00366f;
-    return fdiv(c1 * air_pressure * (1.0f + air_pressure * (60.1f - 0.972f * t) * 1e-10f), 1.0f + c2 * t);
+    return c1 * air_pressure * (1.0f + air_pressure * (60.1f - 0.972f * t) * 1e-10f) / (1.0f + c2 * t);
 }
 __device__ __forceinline__ vec3 refraction(float t_in, float t_out, vec3 v_in, vec3 n) {
     float eta_in = refraction_index(t_in);
@@ -205,7 +202,7 @@ __device__ __forceinline__ vec3 refraction(float t_in, float t_out, vec3 v_in, v
     } else {
         n_ref = -n;
     }
-    float eta = fdiv(eta_in, eta_out);
+    float eta = eta_in / eta_out;
     float k = 1.0f - eta * eta * (1.0f - cosi * cosi);
     if (k < 0.0f) return reflect(v_in, n_ref);  // total internal reflection
     return eta * v_in + (eta * cosi - __builtin_sqrtf(k)) * n_ref;
@@ -214,6 +211,6 @@ __device__ __forceinline__ float point_plane_distance(vec3 p, vec3 n, vec3 p0) {
     float d = dot(p0, n);
     return __builtin_fabsf(dot(p, n) - d);
 }
-__device__ __forceinline__ float sigmoid(float x) { return fdiv(1.0f, 1.0f + ::expf(-x)); }
+__device__ __forceinline__ float sigmoid(float x) { return 1.0f / (1.0f + ::expf(-x)); }
 
 }  // namespace lw
