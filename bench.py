@@ -262,7 +262,7 @@ def bench_batch(verts, faces, replays: int) -> dict:
     view.set_integrator(configs.batch_integrator(max_steps=8000))
     view.set_batch(views)
     t0 = time.perf_counter()
-    tuning, proxy_ms = view.autotune_batch(downscale=4)
+    tuning, tuned_ms = view.autotune_batch(downscale=4)
     tune_s = time.perf_counter() - t0
     view.render_batch()
     view.synchronize()
@@ -296,7 +296,7 @@ def bench_batch(verts, faces, replays: int) -> dict:
             "wall_ms": wall * 1e3, "gsteps_per_s": c["rk4_steps"] / (dev_ms * 1e-3) / 1e9,
             "mrays_per_s": rays / (dev_ms * 1e-3) / 1e6, "plain_kernel_sequential_ms": plain_ms,
             "speedup_vs_plain_sequential": plain_ms / dev_ms if dev_ms > 0 else None,
-            "tuning": tuning.as_dict(), "autotune_proxy_ms": proxy_ms, "autotune_seconds": tune_s}
+            "tuning": tuning.as_dict(), "autotune_full_size_ms": tuned_ms, "autotune_seconds": tune_s}
 
 
 def bench_surrogate(steps: int, fields) -> dict:

@@ -285,9 +285,10 @@ linon_status linon_read_batch_view(linon_context* ctx, int view, uint8_t* rgba, 
 linon_status linon_set_batch_tuning(linon_context* ctx, const linon_batch_tuning* tuning);
 linon_status linon_get_batch_tuning(const linon_context* ctx, linon_batch_tuning* tuning);
 /* Times candidate launch shapes on a copy of the current batch with every view scaled to
- * 1/`downscale` of its width and height, keeps the fastest (coordinate descent over waves per
- * CU, waves per block, steps per compaction round and refill threshold) and re-captures the
- * batch with it.  best_ms: the proxy batch's time with the chosen shape. */
+ * 1/`downscale` of its width and height (coordinate descent over waves per CU, waves per
+ * block, steps per compaction round and refill threshold), times the three fastest once more
+ * on the batch itself, keeps the fastest there and re-captures the batch with it.  best_ms:
+ * the batch's own time with the chosen shape. */
 linon_status linon_autotune_batch(linon_context* ctx, int downscale, linon_batch_tuning* best, float* best_ms);
 
 /* Surrogate (no counterpart in the reference).  linon_surrogate_train generates the data set,
@@ -301,11 +302,14 @@ linon_status linon_surrogate_train(linon_context* ctx, const linon_surrogate_con
 linon_status linon_render_surrogate(linon_context* ctx, void* stream);
 linon_status linon_get_surrogate_stats(linon_context* ctx, linon_surrogate_stats* out);
 
-/* Diagnostics: the shared-denominator division of the IEEE build (lw::div3_rn) against the
- * IEEE division, bit for bit: the reciprocal for every mantissa at exponents -59..59 (both
- * signs), every numerator mantissa at three exponents against 16 structured denominators, and
- * random_pairs random operand pairs (zeros and out-of-range magnitudes included).  Writes the
- * mismatch counts and the number of comparisons. */
+/* Diagnostics: the reciprocal-based divisions of the IEEE build against the IEEE division, bit
+ * for bit: the reciprocal for every mantissa at exponents -126..125 (both signs); the
+ * shared-denominator division (lw::div3_rn) for every numerator mantissa at three exponents
+ * against 16 structured denominators and on random_pairs random operand pairs (zeros and
+ * out-of-range magnitudes included); lw::sigmoid_floor at every float, and hot air's
+ * temperature from it; lw::refraction_in_range against lw::refraction on random_pairs random
+ * temperatures in its range, directions and normals.  quotient_mismatches counts all but the
+ * reciprocal's.  Writes the mismatch counts and the number of comparisons. */
 linon_status linon_selftest_division(int device, uint64_t random_pairs, uint64_t* recip_mismatches,
                                      uint64_t* quotient_mismatches, uint64_t* comparisons);
 /* Diagnostics: the surrogate's BF16-MFMA GEMM on host buffers (bf16 bit patterns in, fp32

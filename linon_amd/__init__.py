@@ -516,6 +516,8 @@ class MainView:
         self._check(self._lib.linon_set_batch_tuning(self._h, C.byref(t)))
 
     def autotune_batch(self, downscale: int = 4):
+        """Searches the launch shape on a 1/downscale proxy of the batch, confirms the three
+        fastest on the batch itself; returns (tuning, the batch's time in ms with it)."""
         t = BatchTuning()
         ms = C.c_float(0)
         self._check(self._lib.linon_autotune_batch(self._h, downscale, C.byref(t), C.byref(ms)))
@@ -557,8 +559,9 @@ class MainView:
 
 
 def selftest_division(random_pairs: int = 1 << 28, device: int = 0) -> dict:
-    """Bit-for-bit check of the IEEE build's shared-denominator division against IEEE division
-    (on the GPU): mismatch counts of the reciprocal and of the quotients, and the number of
+    """Bit-for-bit check of the IEEE build's reciprocal-based divisions against IEEE division
+    (on the GPU): the shared-denominator division, sigmoid_floor and refraction_in_range.
+    Returns the mismatch counts of the reciprocal and of the quotients, and the number of
     comparisons."""
     lib = load_library()
     r, q, n = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)

@@ -68,6 +68,7 @@ linon_status Context::batch_prepare(BatchState& B, const linon_view* views, int 
     LB_CHECK(this, hipMalloc(&B.d_mapping, B.total_px * 16));
     LB_CHECK(this, hipMalloc(&B.d_hit, B.total_px * 4));
     LB_CHECK(this, hipMalloc(&B.d_steps, B.total_px * 4));
+    fill_step(B.step, integ, 1.0f);
     // one group (work queue + persistent launch) per field, views keep their relative order
     std::map<int, std::vector<int>> by_field;
     for (int i = 0; i < n; ++i) by_field[views[i].field].push_back(i);
@@ -126,6 +127,7 @@ linon_status Context::batch_launch(BatchState& B, const linon_batch_tuning& T, h
         P.num_views = g.num_views;
         P.compact_every = std::max(1, T.compact_every);
         P.refill_min = std::max(1, std::min(64, T.refill_min));
+        P.step = B.step;
         P.scene = S;
         LB_CHECK(this, wgsl_math_for(g.field) ? linon_wgslmath::launch_batch(g.field, P, blocks, 64u * (unsigned)wpb, s)
                                                : launch_batch(g.field, P, blocks, 64u * (unsigned)wpb, s));
@@ -247,17 +249,23 @@ linon_status linon_autotune_batch(linon_context* ctx, int downscale, linon_batch
         P.release();
         return c->fail(LINON_ERR_HIP, "autotune_batch: hipEventCreate");
     }
-    auto time_it = [&](const linon_batch_tuning& T, float& ms) -> linon_status {
-        // two runs, keep the second (the first warms caches and the code object)
-        for (int rep = 0; rep < 2; ++rep) {
+    auto time_on = [&](linon::BatchState& B, const linon_batch_tuning& T, int reps, float& ms) -> linon_status {
+        // keep the last of `reps` runs (the first warms caches and the code object)
+        for (int rep = 0; rep < reps; ++rep) {
             if (hipEventRecord(e0, c->stream) != hipSuccess) return c->fail(LINON_ERR_HIP, "autotune: event");
-            linon_status s2 = c->batch_launch(P, T, c->stream);
+            linon_status s2 = c->batch_launch(B, T, c->stream);
             if (s2 != LINON_OK) return s2;
             if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
                 return c->fail(LINON_ERR_HIP, "autotune: synchronize");
             if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return c->fail(LINON_ERR_HIP, "autotune: elapsed");
         }
         return LINON_OK;
+    };
+    std::vector<std::pair<float, linon_batch_tuning>> seen;  // every shape timed on the proxy
+    auto time_it = [&](const linon_batch_tuning& T, float& ms) -> linon_status {
+        linon_status s2 = time_on(P, T, 2, ms);
+        if (s2 == LINON_OK) seen.emplace_back(ms, T);
+        return s2;
     };
     linon_batch_tuning cur = c->tuning;
     float cur_ms = 0.0f;
@@ -286,10 +294,37 @@ linon_status linon_autotune_batch(linon_context* ctx, int downscale, linon_batch
             }
         }
     }
+    P.release();
+    // the proxy ranks shapes only roughly at full size: the three fastest on the proxy are
+    // timed once more on the batch itself, and the fastest there is kept
+    if (st == LINON_OK && !seen.empty()) {
+        std::stable_sort(seen.begin(), seen.end(),
+                         [](const auto& a, const auto& b) { return a.first < b.first; });
+        auto same = [](const linon_batch_tuning& a, const linon_batch_tuning& b) {
+            return a.waves_per_cu == b.waves_per_cu && a.waves_per_block == b.waves_per_block &&
+                   a.compact_every == b.compact_every && a.refill_min == b.refill_min;
+        };
+        std::vector<linon_batch_tuning> finalists;
+        for (const auto& e : seen) {
+            bool dup = false;
+            for (const auto& f : finalists) dup = dup || same(f, e.second);
+            if (!dup) finalists.push_back(e.second);
+            if (finalists.size() == 3) break;
+        }
+        float best_full = 0.0f;
+        for (size_t i = 0; i < finalists.size() && st == LINON_OK; ++i) {
+            float ms = 0.0f;
+            st = time_on(c->batch, finalists[i], i == 0 ? 2 : 1, ms);
+            if (st == LINON_OK && (i == 0 || ms < best_full)) {
+                best_full = ms;
+                cur = finalists[i];
+                cur_ms = ms;
+            }
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipDeviceSynchronize();
-    P.release();
     if (st != LINON_OK) return st;
     c->tuning = cur;
     if (best) *best = cur;

@@ -96,6 +96,7 @@ void Context::fill_scene(SceneParams& S) const {
         S.grid_lo[k] = grid_lo[k];
         S.grid_hi[k] = grid_hi[k];
         S.gdim[k] = gdim[k];
+        S.gdimf[k] = (float)gdim[k];
     }
     S.cell_size = cell_size;
     S.inv_cell_size = 1.0f / cell_size;

@@ -29,6 +29,7 @@ struct BatchState {
     float* d_mapping = nullptr;
     int* d_hit = nullptr;
     int* d_steps = nullptr;
+    StepParams step{};              // the integrator at linon_set_batch (views add h and weight)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void release();

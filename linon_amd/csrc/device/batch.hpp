@@ -15,6 +15,13 @@
 
 namespace lk {
 
+// One lane's tallies over the rays it traced (LDS; the launch reserves blockDim.x of them).
+struct BatchLaneTally {
+    QueryStats q;
+    unsigned long long steps;
+    unsigned rays, pad;
+};
+
 // largest v with views[v].first_ray <= ray
 __device__ __forceinline__ int find_view(const BatchView* views, int n, unsigned long long ray) {
     int lo = 0, hi = n - 1;
@@ -32,14 +39,21 @@ template <class F>
 __device__ void batch_body(const BatchParams& B) {
     const int lane = threadIdx.x & 63;
     RayState r;
-    ray_begin(r, vec3f(0.0f, 0.0f, 0.0f), vec3f(0.0f, 0.0f, 1.0f), StepParams{1.0f, 0.0f, 1.0f, 0.0f, 0, 0, 0, 0});
-    StepParams sp{1.0f, 0.0f, 1.0f, 0.0f, 0, 0, 0, 0};
+    ray_begin(r, vec3f(0.0f, 0.0f, 0.0f), vec3f(0.0f, 0.0f, 1.0f), B.step);
+    // per lane only the view's step size and field weight; the rest of the loop's parameters
+    // are the batch's (wave-uniform, so they stay in scalar registers)
+    float lane_h = 1.0f, lane_w = 1.0f;
     bool busy = false;
     int view = 0;
-    size_t texel = 0;
+    unsigned texel = 0;  // views are at most 16384 x 16384
     bool exhausted = false;
-    unsigned long long steps = 0, rays = 0, refills = 0;
-    QueryStats st{0ull, 0u, 0u, 0u};
+    unsigned long long refills = 0;
+    // per-lane tallies in LDS (blockDim.x entries, sized by the launch), out of the registers
+    // the step loop needs
+    extern __shared__ BatchLaneTally batch_tally[];
+    BatchLaneTally& tally = batch_tally[threadIdx.x];
+    QueryStats& st = tally.q;
+    tally = BatchLaneTally{QueryStats{0ull, 0u, 0u, 0u}, 0ull, 0u, 0u};
     while (true) {
         // ---- refill idle lanes (wave-uniform decision) ----
         const unsigned long long idle = __ballot(!busy);
@@ -61,9 +75,13 @@ __device__ void batch_body(const BatchParams& B) {
                     const int gy = (tile / V.tiles_x) * LINON_TILE + (l / LINON_TILE);
                     if (gx < V.cam.width && gy < V.cam.height) {
                         const CameraFrame cam = camera_frame(V.cam);
-                        sp = V.step;
+                        lane_h = V.step.h_initial;
+                        lane_w = V.step.field_weight;
+                        StepParams sp = B.step;
+                        sp.h_initial = lane_h;
+                        sp.field_weight = lane_w;
                         ray_begin(r, cam.origin, camera_dir(cam, V.cam, gx, gy), sp);
-                        texel = (size_t)(V.cam.height - 1 - gy) * V.cam.width + gx;
+                        texel = (unsigned)(V.cam.height - 1 - gy) * (unsigned)V.cam.width + (unsigned)gx;
                         busy = true;
                     }
                 }
@@ -75,20 +93,19 @@ __device__ void batch_body(const BatchParams& B) {
         }
         // ---- advance the lanes' rays ----
         if (busy) {
-            if (ray_advance<F>(r, sp, B.scene, nullptr, B.compact_every)) {
+            StepParams sp = B.step;
+            sp.h_initial = lane_h;
+            sp.field_weight = lane_w;
+            if (ray_advance<F>(r, sp, B.scene, nullptr, B.compact_every, st)) {
                 const BatchView& V = B.views[view];
                 ray_store(r, texel, V.color, V.mapping, V.hit_face, V.steps);
-                steps += (unsigned long long)r.iters;
-                st.tests += r.st.tests;
-                st.cells += r.st.cells;
-                st.queries += r.st.queries;
-                st.skips += r.st.skips;
-                rays += 1;
+                tally.steps += (unsigned long long)r.iters;
+                tally.rays += 1u;
                 busy = false;
             }
         }
     }
-    count_wave(B.counters, lane, steps, st, rays, lane == 0 ? refills : 0ull);
+    count_wave(B.counters, lane, tally.steps, st, (unsigned long long)tally.rays, lane == 0 ? refills : 0ull);
 }
 
 }  // namespace lk

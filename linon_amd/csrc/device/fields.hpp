@@ -31,15 +31,16 @@ struct FieldMirage {  // src/functions.rs:36-96
         }
         float part_in, part_out;
         if (Sigmoid) {
-            part_in = clamp(0.0f, 1.0f, sigmoid(dist_in / max_dist * 12.0f - 6.0f));
-            part_out = clamp(0.0f, 1.0f, sigmoid(dist_out / max_dist * 12.0f - 6.0f));
+            // arguments >= -6 (distances are >= 0): sigmoid_floor equals sigmoid here
+            part_in = clamp(0.0f, 1.0f, sigmoid_floor(dist_in / max_dist * 12.0f - 6.0f));
+            part_out = clamp(0.0f, 1.0f, sigmoid_floor(dist_out / max_dist * 12.0f - 6.0f));
         } else {
             part_in = clamp(0.0f, 1.0f, dist_in / max_dist);
             part_out = clamp(0.0f, 1.0f, dist_out / max_dist);
         }
         float t_in = part_in * t_env + (1.0f - part_in) * t_src;
         float t_out = part_out * t_env + (1.0f - part_out) * t_src;
-        return refraction(t_in, t_out, v, normal);
+        return refraction_in_range(t_in, t_out, v, normal);  // t_in, t_out in [15, 200]
     }
 };
 using FieldMirageSpherical = FieldMirage<true, false>;
@@ -91,11 +92,11 @@ struct FieldLens {  // v - 0.3 * p / max(|p|^3, 1e-3)
 struct FieldHotAir {  // refraction(T(p_prev), T(p), v, normalize(p - c)), T = 20 + 80 sigmoid(8 (0.5 - |p - c|))
     __device__ __forceinline__ static float temperature(vec3 q) {
         const vec3 c = vec3f(0.0f, -0.5f, 0.0f);
-        return 20.0f + 80.0f * sigmoid(8.0f * (0.5f - length(q - c)));
+        return 20.0f + 80.0f * sigmoid_floor(8.0f * (0.5f - length(q - c)));  // in [20, 100]
     }
     __device__ __forceinline__ vec3 operator()(vec3 p_prev, vec3 p, vec3, vec3 v, float) const {
         const vec3 c = vec3f(0.0f, -0.5f, 0.0f);
-        return refraction(temperature(p_prev), temperature(p), v, normalize(p - c));
+        return refraction_in_range(temperature(p_prev), temperature(p), v, normalize(p - c));
     }
 };
 struct FieldWarp {  // v0 + 0.3 * (sin 4p.y, 0, sin 4p.x)

@@ -34,6 +34,7 @@ struct SceneParams {
     float cell_size, inv_cell_size;
     float skip_margin;
     int gdim[3];
+    float gdimf[3];  // gdim as floats, so that kernels compare against scalar-register arguments
 };
 
 // The integration loop of one view (src/main_view.wgsl:229-279 plus the escape mode).
@@ -120,6 +121,8 @@ struct BatchParams {
     int compact_every;  // RK4 iterations a wave runs between refills
     int refill_min;     // refill once at least this many lanes are idle
     int pad;
+    StepParams step;    // the integration loop shared by all views; each view's h_initial and
+                        // field_weight (views[v].step) replace those two
     SceneParams scene;
 };
 

@@ -17,6 +17,30 @@
 #include "params.hpp"
 #include "wgsl_lib.hpp"
 
+// Waves per SIMD the ray-cast kernels (static and run-time compiled) are built for.  6 caps a
+// kernel at 80 VGPRs; the compiler's own choice is 94-103 (4-5 waves).  At 6 a few values
+// spill outside the step loop, and the four 1080p benchmark fields render 5-12% faster than at
+// 4-5 waves, as fast as at 7 and slower at 8.  0 leaves the choice to the compiler.
+#ifndef LK_RAYCAST_WAVES_PER_EU
+#define LK_RAYCAST_WAVES_PER_EU 6
+#endif
+#if LK_RAYCAST_WAVES_PER_EU > 0
+#define LK_RAYCAST_OCCUPANCY __attribute__((amdgpu_waves_per_eu(LK_RAYCAST_WAVES_PER_EU)))
+#else
+#define LK_RAYCAST_OCCUPANCY
+#endif
+// The same for the persistent batch kernel (256-lane blocks): the compiler's own 121-131 VGPRs
+// allow 3-4 waves; at 6 (80 VGPRs) the 65-view batch runs 10% faster, at 7 slower again and at
+// 8 twice as slow.
+#ifndef LK_BATCH_WAVES_PER_EU
+#define LK_BATCH_WAVES_PER_EU 6
+#endif
+#if LK_BATCH_WAVES_PER_EU > 0
+#define LK_BATCH_OCCUPANCY __attribute__((amdgpu_waves_per_eu(LK_BATCH_WAVES_PER_EU)))
+#else
+#define LK_BATCH_OCCUPANCY
+#endif
+
 namespace lk {
 
 using lw::vec3;
@@ -66,6 +90,20 @@ __device__ __forceinline__ vec3 normalize(vec3 a) {
 }
 }  // namespace rn
 
+// An opaque copy of x: the compiler cannot see that it equals x, so no operation on the copy
+// is merged (common-subexpression elimination) with an operation on x.  The integrator passes
+// the field its arguments this way.  The field's own arithmetic may be contracted into FMAs
+// while the integrator's rn:: arithmetic may not; were a product of the field merged with the
+// same product of the integrator (say p.x * p.x of length(p) and of the escape test's
+// dot(p, p)), the merged product would lose its contraction, and the field's rounding would
+// depend on the kernel around it.  With opaque arguments every kernel that integrates a ray
+// rounds the field alike.
+__device__ __forceinline__ float opaque(float x) {
+    asm("" : "+v"(x));
+    return x;
+}
+__device__ __forceinline__ vec3 opaque(vec3 a) { return vec3f(opaque(a.x), opaque(a.y), opaque(a.z)); }
+
 // Moeller-Trumbore in the operation order of hit_triangle (src/main_view.wgsl:150-175):
 // two-sided, |a| < 1e-7 rejected, inclusive edges, t > 1e-7 accepted.  e1 = v1 - v0 and
 // e2 = v2 - v0 are precomputed in f32, which gives the same bits as computing them here.
@@ -95,6 +133,8 @@ __device__ __forceinline__ void consider(float t_new, int face, float max_dist, 
     }
 }
 
+// Scene-query counters of one lane, accumulated over the rays it traces.  The kernels keep
+// them in LDS (one slot per lane) rather than in registers, which the step loop needs more.
 struct QueryStats {
     unsigned long long tests;
     unsigned cells;
@@ -111,7 +151,7 @@ __device__ __forceinline__ float grid_safe_distance(const SceneParams& P, vec3 p
     float gx = (p.x - P.grid_lo[0]) * ics;
     float gy = (p.y - P.grid_lo[1]) * ics;
     float gz = (p.z - P.grid_lo[2]) * ics;
-    const float nx = (float)P.gdim[0], ny = (float)P.gdim[1], nz = (float)P.gdim[2];
+    const float nx = P.gdimf[0], ny = P.gdimf[1], nz = P.gdimf[2];
     if (!(gx >= 0.0f && gx < nx && gy >= 0.0f && gy < ny && gz >= 0.0f && gz < nz)) {
         float dx = fmaxf(fmaxf(-gx, gx - nx), 0.0f);
         float dy = fmaxf(fmaxf(-gy, gy - ny), 0.0f);
@@ -156,16 +196,16 @@ __device__ __forceinline__ void query_grid(const SceneParams& P, vec3 o, vec3 d,
     // clip the segment to the grid box
 #define LK_CLIP(a, dd, n)                                              \
     if (dd == 0.0f) {                                                  \
-        if (!(a >= 0.0f && a <= (float)n)) return;                     \
+        if (!(a >= 0.0f && a <= n)) return;                            \
     } else {                                                           \
         float inv = 1.0f / dd;                                         \
-        float ta = (0.0f - a) * inv, tb = ((float)n - a) * inv;        \
+        float ta = (0.0f - a) * inv, tb = (n - a) * inv;               \
         s0 = fmaxf(s0, fminf(ta, tb));                                 \
         s1 = fminf(s1, fmaxf(ta, tb));                                 \
     }
-    LK_CLIP(ax, dx, nx)
-    LK_CLIP(ay, dy, ny)
-    LK_CLIP(az, dz, nz)
+    LK_CLIP(ax, dx, P.gdimf[0])
+    LK_CLIP(ay, dy, P.gdimf[1])
+    LK_CLIP(az, dz, P.gdimf[2])
 #undef LK_CLIP
     if (!(s0 <= s1)) return;
     int cx = min(max((int)floorf(ax + dx * s0), 0), nx - 1);
@@ -378,10 +418,8 @@ struct RayState {
     vec3 p, d, v0, last_v;
     float t, h, last_diff;
     float safe;  // grid path: lower bound on the distance from p to the mesh
-    float cr, cg, cb;
-    int face, iters;
-    bool has_color;
-    QueryStats st;
+    unsigned color;  // rgba8 of the hit's shading (alpha 1), packed at the hit; black before
+    int face, iters;  // face >= 0 once the ray has its colour
 };
 
 __device__ __forceinline__ void ray_begin(RayState& r, vec3 origin, vec3 dir, const StepParams& I) {
@@ -393,11 +431,9 @@ __device__ __forceinline__ void ray_begin(RayState& r, vec3 origin, vec3 dir, co
     r.h = I.h_initial;
     r.last_diff = -1.0f;
     r.safe = 0.0f;
-    r.cr = r.cg = r.cb = 0.0f;
+    r.color = 0xff000000u;
     r.face = -1;
     r.iters = 0;
-    r.has_color = false;
-    r.st = QueryStats{0ull, 0u, 0u, 0u};
 }
 
 __device__ __forceinline__ bool ray_finished(const RayState& r, const StepParams& I) {
@@ -408,10 +444,10 @@ __device__ __forceinline__ bool ray_finished(const RayState& r, const StepParams
 // RK4 with the previous stage passed as `v`, the adaptive redo at h = 0.001, the blend with
 // field_weight, the segment query until the first hit, and the escape tests of the escape
 // mode.  Returns true once the ray is finished.  `lds` holds the faces when the brute-force
-// path staged them, else null.
+// path staged them, else null; the scene queries are counted into `st`.
 template <class F>
 __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, const SceneParams& S,
-                                            const float4* lds, int budget) {
+                                            const float4* lds, int budget, QueryStats& st) {
     const F field{};
     const bool adaptive = I.adaptive != 0;
     const bool escape = (I.mode == LINON_MODE_ESCAPE);
@@ -426,10 +462,13 @@ __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, co
         const float h = r.h;
         const float hh = rn::mul(0.5f, h);
         const vec3 p = r.p;
-        vec3 k1 = field(p, p, start_dir, r.d, r.t);
-        vec3 k2 = field(p, rn::add(p, rn::mul(hh, k1)), start_dir, k1, rn::add(r.t, hh));
-        vec3 k3 = field(p, rn::add(p, rn::mul(hh, k2)), start_dir, k2, rn::add(r.t, hh));
-        vec3 k4 = field(p, rn::add(p, rn::mul(h, k3)), start_dir, k3, rn::add(r.t, h));
+        // the field sees opaque copies of the ray state the integrator itself computes with
+        // (position and direction; see opaque())
+        const vec3 pf = opaque(p);
+        vec3 k1 = field(pf, pf, start_dir, opaque(r.d), r.t);
+        vec3 k2 = field(pf, rn::add(p, rn::mul(hh, k1)), start_dir, k1, rn::add(r.t, hh));
+        vec3 k3 = field(pf, rn::add(p, rn::mul(hh, k2)), start_dir, k2, rn::add(r.t, hh));
+        vec3 k4 = field(pf, rn::add(p, rn::mul(h, k3)), start_dir, k3, rn::add(r.t, h));
         vec3 V = rk4_average(k1, k2, k3, k4);
         float diff = 0.0f;
         if (adaptive) {  // src/main_view.wgsl:251-255
@@ -441,23 +480,23 @@ __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, co
         }
         r.d = rn::add(rn::mul(rn::sub(1.0f, w), r.d), rn::mul(w, V));
         const vec3 step = rn::mul(r.d, h);
-        if (!r.has_color) {
+        if (r.face < 0) {
             const float L = rn::length(step);
             bool query = (L > 0.0f && L < __builtin_inff());  // normalize(0) or NaN: no hit
             if (query && !brute) {
                 if (!(r.safe > L + margin)) r.safe = grid_safe_distance(S, p);
                 query = !(r.safe > L + margin);
-                r.st.skips += query ? 0u : 1u;
+                st.skips += query ? 0u : 1u;
                 r.safe -= L;
             }
             if (query) {
-                r.st.queries += 1;
+                st.queries += 1;
                 const vec3 nd = step / L;  // normalize(step_dir)
                 float best_t = -1.0f;
                 int best_face = -1;
                 if (brute) {
                     const int nf = S.num_faces;
-                    r.st.tests += (unsigned long long)nf;
+                    st.tests += (unsigned long long)nf;
                     if (lds) {
                         for (int f = 0; f < nf; ++f) {
                             float4 a = lds[3 * f], e1 = lds[3 * f + 1], e2 = lds[3 * f + 2];
@@ -473,10 +512,9 @@ __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, co
                         }
                     }
                 } else {
-                    query_grid(S, p, nd, L, best_t, best_face, r.st);
+                    query_grid(S, p, nd, L, best_t, best_face, st);
                 }
                 if (best_face >= 0) {  // shading of ray_color, src/main_view.wgsl:199-218
-                    r.has_color = true;
                     r.face = best_face;
                     vec3 e1 = ld_tri_vec(S.tris, best_face, 1), e2 = ld_tri_vec(S.tris, best_face, 2);
                     vec3 n = normalize(cross(e1, e2));
@@ -484,9 +522,7 @@ __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, co
                     if (intensity == 0.0f) intensity = fmaxf(dot(-n, -nd), 0.0f);
                     float spec = powf(intensity, 64.0f);
                     float s = (0.01f + intensity) + spec;
-                    r.cr = s * fabsf(n.x);
-                    r.cg = s * fabsf(n.y);
-                    r.cb = s * fabsf(n.z);
+                    r.color = pack_rgba8(s * fabsf(n.x), s * fabsf(n.y), s * fabsf(n.z), 1.0f);
                 }
             }
         }
@@ -498,7 +534,7 @@ __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, co
             r.last_diff = diff;
         }
         if (escape) {
-            if (r.has_color && !I.need_mapping) return true;
+            if (r.face >= 0 && !I.need_mapping) return true;
             if (rn::dot(r.p, r.p) > I.escape_r2 && rn::dot(r.p, r.d) > 0.0f) return true;
         }
     }
@@ -509,7 +545,7 @@ __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, co
 // (src/main_view.wgsl:273-277, 372-374).
 __device__ __forceinline__ void ray_store(const RayState& r, size_t texel, unsigned* color, float* mapping,
                                           int* hit_face, int* steps) {
-    color[texel] = pack_rgba8(r.cr, r.cg, r.cb, 1.0f);
+    color[texel] = r.color;
     reinterpret_cast<float4*>(mapping)[texel] = make_float4(r.p.x, r.p.y, r.p.z, 1.0f);
     if (hit_face) hit_face[texel] = r.face;
     if (steps) steps[texel] = r.iters;
@@ -561,14 +597,22 @@ __device__ __forceinline__ void raycast_body(const RaycastParams& P) {
     // lanes outside the image stay in the wave (zero work) so the wave reductions below see
     // every lane
     const bool active = gx < P.cam.width && gy < P.cam.height;
+    __shared__ QueryStats lds_stats[64];
+    lds_stats[lane] = QueryStats{0ull, 0u, 0u, 0u};
 
     RayState r;
     ray_begin(r, cam.origin, camera_dir(cam, P.cam, gx, gy), P.step);
-    if (active) {
-        ray_advance<F>(r, P.step, P.scene, brute_lds ? lds_tris : nullptr, 0x7fffffff);
-        ray_store(r, (size_t)(P.cam.height - 1 - gy) * P.cam.width + gx, P.color, P.mapping, P.hit_face, P.steps);
-    }
-    count_wave(P.counters, lane, (unsigned long long)r.iters, r.st, active ? 1ull : 0ull, 0ull);
+    if (active) ray_advance<F>(r, P.step, P.scene, brute_lds ? lds_tris : nullptr, 0x7fffffff, lds_stats[lane]);
+    // The pixel again, from the hardware lane count rather than the work-item id, so that no
+    // index has to stay in a register (or spill) across the step loop.
+    const int l = (int)__lane_id();
+    const int x = (tile % P.tiles_x) * LINON_TILE + (l & (LINON_TILE - 1));
+    const int y = (tile / P.tiles_x) * LINON_TILE + (l / LINON_TILE);
+    const bool stored = x < P.cam.width && y < P.cam.height;
+    // row-flipped texel (the host keeps W and H <= 16384, so it fits 32 bits)
+    if (stored) ray_store(r, (unsigned)(P.cam.height - 1 - y) * (unsigned)P.cam.width + (unsigned)x, P.color,
+                          P.mapping, P.hit_face, P.steps);
+    count_wave(P.counters, l, (unsigned long long)r.iters, lds_stats[l], stored ? 1ull : 0ull, 0ull);
 }
 
 // src/overlay.wgsl:99-146, one lane per texel.

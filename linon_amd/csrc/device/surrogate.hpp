@@ -92,8 +92,9 @@ __device__ void approach_body(const ApproachParams& A) {
     const float r2 = A.radius * A.radius;
     bool entered = lk::rn::dot(r.p, r.p) < r2;
     bool done = false;
+    lk::QueryStats st{0ull, 0u, 0u, 0u};  // no mesh: nothing is counted
     while (!entered && !done) {
-        done = lk::ray_advance<F>(r, A.step, none, nullptr, 1);
+        done = lk::ray_advance<F>(r, A.step, none, nullptr, 1, st);
         entered = lk::rn::dot(r.p, r.p) < r2;
     }
     const size_t texel = (size_t)(A.cam.height - 1 - gy) * A.cam.width + gx;
@@ -134,7 +135,8 @@ __device__ void flowmap_body(const FlowmapParams& P) {
     none.num_faces = 0;
     none.tris = nullptr;
     none.skip_margin = 0.0f;
-    lk::ray_advance<F>(r, P.step, none, nullptr, 0x7fffffff);
+    lk::QueryStats st{0ull, 0u, 0u, 0u};  // no mesh: nothing is counted
+    lk::ray_advance<F>(r, P.step, none, nullptr, 0x7fffffff, st);
     float* o = P.exits + 6 * (size_t)slot;
     o[0] = r.p.x; o[1] = r.p.y; o[2] = r.p.z;
     o[3] = r.d.x; o[4] = r.d.y; o[5] = r.d.z;

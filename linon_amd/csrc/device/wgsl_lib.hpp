@@ -64,6 +64,17 @@ __device__ __forceinline__ vec3 operator/(vec3 a, float s) { return {a.x / s, a.
 #else
 __device__ __forceinline__ vec3 operator/(vec3 a, float s) { return div3_rn(a, s); }
 #endif
+// Scalar quotients whose operands are in [2^-60, 2^60] (or NaN) by construction, so that the
+// correctly rounded reciprocal-and-correction needs neither a range test nor the ordinary
+// division beside it (each caller states why its operands are in range).  A guarded scalar
+// division costs more in tests and branches than it saves, so the general helpers keep `/`.
+__device__ __forceinline__ float div_in_range(float a, float b) {
+#ifdef LINON_FAST_MATH
+    return a / b;
+#else
+    return quot_rn(a, b, recip_rn(b));
+#endif
+}
 __device__ __forceinline__ vec3 operator/(float s, vec3 a) { return {s / a.x, s / a.y, s / a.z}; }
 __device__ __forceinline__ vec3 operator+(vec3 a, float s) { return {a.x + s, a.y + s, a.z + s}; }
 __device__ __forceinline__ vec3 operator+(float s, vec3 a) { return {s + a.x, s + a.y, s + a.z}; }
@@ -185,16 +196,26 @@ __device__ __forceinline__ vec3 translate(vec3 v, float dx, float dy, float dz) 
     return {v.x + dx, v.y + dy, v.z + dz};
 }
 // Air refraction index at temperature t (deg C), Zhao et al.; pow(10.0, -10.0) is a
-// constant expression and enters as 1e-10.
-__device__ __forceinline__ float refraction_index(float t) {
+// constant expression and enters as 1e-10.  Numerator and denominator are separate helpers so
+// that both quotient forms below round the same operands.
+__device__ __forceinline__ float refraction_index_num(float t) {
     const float air_pressure = 101325.0f;
     const float c1 = 0.0000104f;
-    const float c2 = 0.00366f;
-    return c1 * air_pressure * (1.0f + air_pressure * (60.1f - 0.972f * t) * 1e-10f) / (1.0f + c2 * t);
+    return c1 * air_pressure * (1.0f + air_pressure * (60.1f - 0.972f * t) * 1e-10f);
 }
-__device__ __forceinline__ vec3 refraction(float t_in, float t_out, vec3 v_in, vec3 n) {
-    float eta_in = refraction_index(t_in);
-    float eta_out = refraction_index(t_out);
+__device__ __forceinline__ float refraction_index_den(float t) {
+    const float c2 = 0.00366f;
+    return 1.0f + c2 * t;
+}
+__device__ __forceinline__ float refraction_index(float t) { return refraction_index_num(t) / refraction_index_den(t); }
+// InRange: temperatures in [-250, 1e4] deg C (or NaN).  There the index's numerator lies in
+// [0.95, 1.06] and its denominator in [0.085, 37.6], so both indices lie in [0.025, 12.5] and
+// every quotient has its operands in div_in_range's range.
+template <bool InRange>
+__device__ __forceinline__ vec3 refraction_impl(float t_in, float t_out, vec3 v_in, vec3 n) {
+    auto quot = [](float a, float b) { return InRange ? div_in_range(a, b) : a / b; };
+    float eta_in = quot(refraction_index_num(t_in), refraction_index_den(t_in));
+    float eta_out = quot(refraction_index_num(t_out), refraction_index_den(t_out));
     float cosi = clamp(-1.0f, 1.0f, dot(v_in, n));
     vec3 n_ref = n;
     if (cosi < 0.0f) {
@@ -202,15 +223,35 @@ __device__ __forceinline__ vec3 refraction(float t_in, float t_out, vec3 v_in, v
     } else {
         n_ref = -n;
     }
-    float eta = eta_in / eta_out;
+    float eta = quot(eta_in, eta_out);
     float k = 1.0f - eta * eta * (1.0f - cosi * cosi);
     if (k < 0.0f) return reflect(v_in, n_ref);  // total internal reflection
     return eta * v_in + (eta * cosi - __builtin_sqrtf(k)) * n_ref;
+}
+__device__ __forceinline__ vec3 refraction(float t_in, float t_out, vec3 v_in, vec3 n) {
+    return refraction_impl<false>(t_in, t_out, v_in, n);
+}
+// refraction() for temperatures in [-250, 1e4] deg C (the predefined fields': hot air's lie in
+// [20, 100], the mirages' in [15, 200]); bit for bit the same result.
+__device__ __forceinline__ vec3 refraction_in_range(float t_in, float t_out, vec3 v_in, vec3 n) {
+    return refraction_impl<true>(t_in, t_out, v_in, n);
 }
 __device__ __forceinline__ float point_plane_distance(vec3 p, vec3 n, vec3 p0) {
     float d = dot(p0, n);
     return __builtin_fabsf(dot(p, n) - d);
 }
 __device__ __forceinline__ float sigmoid(float x) { return 1.0f / (1.0f + ::expf(-x)); }
+// sigmoid(x) with its reciprocal taken of min(1 + e^-x, 2^125): the operand is then in
+// [1, 2^125] (or NaN), so the correctly rounded reciprocal needs no range test.  Equal to
+// sigmoid(x) wherever sigmoid(x) >= 2^-125, and 2^-125 below, for callers to which the two are
+// indistinguishable (hot air's 20 + 80 s rounds to 20 for any s below 2^-118).
+__device__ __forceinline__ float sigmoid_floor(float x) {
+#ifdef LINON_FAST_MATH
+    return sigmoid(x);
+#else
+    const float b = 1.0f + ::expf(-x);
+    return recip_rn(b > 0x1p125f ? 0x1p125f : b);
+#endif
+}
 
 }  // namespace lw

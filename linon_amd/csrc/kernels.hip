@@ -11,12 +11,12 @@
 namespace LINON_KERNEL_NS {
 
 template <class F>
-__global__ void __launch_bounds__(64) raycast_kernel(RaycastParams P) {
+__global__ void __launch_bounds__(64) LK_RAYCAST_OCCUPANCY raycast_kernel(RaycastParams P) {
     lk::raycast_body<F>(P);
 }
 
 template <class F>
-__global__ void __launch_bounds__(256) batch_kernel(BatchParams B) {
+__global__ void __launch_bounds__(256) LK_BATCH_OCCUPANCY batch_kernel(BatchParams B) {
     lk::batch_body<F>(B);
 }
 
@@ -59,7 +59,7 @@ hipError_t launch_batch(int field, const BatchParams& B, unsigned blocks, unsign
     const void* fn = batch_kernel_for(field);
     if (!fn) return hipErrorInvalidDeviceFunction;
     void* args[] = {const_cast<BatchParams*>(&B)};
-    return hipLaunchKernel(fn, dim3(blocks), dim3(threads), args, 0, s);
+    return hipLaunchKernel(fn, dim3(blocks), dim3(threads), args, threads * sizeof(lk::BatchLaneTally), s);
 }
 
 const void* raycast_kernel_for(int id) {
@@ -108,7 +108,8 @@ hipError_t launch_clear_u64(unsigned long long* p, int n, hipStream_t s) {
     return hipLaunchKernel((const void*)&clear_u64_kernel, dim3((n + 63) / 64), dim3(64), args, 0, s);
 }
 
-// ---- self-test of lw::div3_rn / lw::div_rn against the IEEE division (IEEE build only) ----
+// ---- self-test of lw::div3_rn / lw::div_rn and the in-range quotient helpers against the IEEE
+// division (IEEE build only) ----
 #ifndef LINON_FAST_MATH
 __device__ __forceinline__ unsigned fbits(float x) { return __float_as_uint(x); }
 
@@ -119,8 +120,28 @@ __global__ void __launch_bounds__(256) div_selftest_recip_kernel(int e, unsigned
     unsigned bad = 0;
     for (int sgn = 0; sgn < 2; ++sgn) {
         const float b = __uint_as_float(((unsigned)sgn << 31) | ((unsigned)(e + 127) << 23) | m);
-        if (lw::div_rn_range(b) && fbits(lw::recip_rn(b)) != fbits(1.0f / b)) ++bad;
+        if (fbits(lw::recip_rn(b)) != fbits(1.0f / b)) ++bad;
     }
+    if (bad) atomicAdd(mism, (unsigned long long)bad);
+}
+
+// every float x whose bit pattern has the top byte `hi`: sigmoid_floor(x) equals sigmoid(x)
+// where that is >= 2^-125 (2^-125 below; NaN for NaN), and hot air's 20 + 80 s (fused and
+// unfused) is the same for both
+__device__ __forceinline__ bool same_or_nan(float a, float b) { return fbits(a) == fbits(b) || (a != a && b != b); }
+__device__ __forceinline__ float temp_unfused(float s) {
+#pragma clang fp contract(off)
+    return 20.0f + 80.0f * s;
+}
+__global__ void __launch_bounds__(256) div_selftest_sigmoid_kernel(unsigned hi, unsigned long long* mism) {
+    const unsigned lo = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lo >= (1u << 24)) return;
+    const float x = __uint_as_float((hi << 24) | lo);
+    const float ref = lw::sigmoid(x), s = lw::sigmoid_floor(x);
+    unsigned bad = 0;
+    bad += !same_or_nan(s, ref >= 0x1p-125f || ref != ref ? ref : 0x1p-125f);
+    bad += !same_or_nan(__builtin_fmaf(80.0f, s, 20.0f), __builtin_fmaf(80.0f, ref, 20.0f));
+    bad += !same_or_nan(temp_unfused(s), temp_unfused(ref));
     if (bad) atomicAdd(mism, (unsigned long long)bad);
 }
 
@@ -167,6 +188,29 @@ __global__ void __launch_bounds__(256) div_selftest_random_kernel(unsigned long 
     bad += fbits(lw::div_rn(a, b)) != fbits(a / b);
     if (bad) atomicAdd(mism, (unsigned long long)bad);
 }
+
+// refraction_in_range against refraction: temperatures uniform in [-250, 1e4] (one in 64 at an
+// end of the range, one in 1024 NaN), directions and normals with components in [-1.5, 1.5]
+__device__ __forceinline__ float rand_unit(unsigned long long r) { return (float)(r >> 40) * 0x1p-24f; }
+__device__ __forceinline__ float rand_temp(unsigned long long r) {
+    const unsigned sel = (unsigned)(r & 1023u);
+    if (sel == 0) return __uint_as_float(0x7fc00000u);
+    if (sel < 16) return (sel & 1) ? -250.0f : 1e4f;
+    return -250.0f + 10250.0f * rand_unit(r);
+}
+__global__ void __launch_bounds__(256) div_selftest_refraction_kernel(unsigned long long seed, unsigned long long n,
+                                                                      unsigned long long* mism) {
+    const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long r[8];
+    for (int k = 0; k < 8; ++k) r[k] = smix(seed + 0x9E3779B97F4A7C15ull * (8 * i + k + 1));
+    const float t_in = rand_temp(r[0]), t_out = rand_temp(r[1]);
+    const lw::vec3 v = lw::vec3f(3.0f * rand_unit(r[2]) - 1.5f, 3.0f * rand_unit(r[3]) - 1.5f, 3.0f * rand_unit(r[4]) - 1.5f);
+    const lw::vec3 nrm = lw::vec3f(3.0f * rand_unit(r[5]) - 1.5f, 3.0f * rand_unit(r[6]) - 1.5f, 3.0f * rand_unit(r[7]) - 1.5f);
+    const lw::vec3 a = lw::refraction(t_in, t_out, v, nrm), b = lw::refraction_in_range(t_in, t_out, v, nrm);
+    const unsigned bad = !same_or_nan(a.x, b.x) + !same_or_nan(a.y, b.y) + !same_or_nan(a.z, b.z);
+    if (bad) atomicAdd(mism, (unsigned long long)bad);
+}
 #endif
 
 #ifndef LINON_FAST_MATH
@@ -178,7 +222,8 @@ hipError_t run_division_selftest(unsigned long long random_pairs, unsigned long 
     e = hipMemset(d, 0, 2 * sizeof(unsigned long long));
     unsigned long long n = 0;
     const unsigned mblocks = (1u << 23) / 256;
-    for (int ex = -59; ex <= 59 && e == hipSuccess; ++ex) {  // every mantissa, both signs
+    // every mantissa, both signs, at every exponent whose reciprocal is normal
+    for (int ex = -126; ex <= 125 && e == hipSuccess; ++ex) {
         hipLaunchKernelGGL(div_selftest_recip_kernel, dim3(mblocks), dim3(256), 0, nullptr, ex, d);
         n += 2ull << 23;
     }
@@ -192,8 +237,18 @@ hipError_t run_division_selftest(unsigned long long random_pairs, unsigned long 
             hipLaunchKernelGGL(div_selftest_mantissa_kernel, dim3(mblocks), dim3(256), 0, nullptr, ea, b, d + 1);
             n += 3ull << 23;
         }
+    for (unsigned hi = 0; hi < 256 && e == hipSuccess; ++hi) {  // sigmoid_floor at every float
+        hipLaunchKernelGGL(div_selftest_sigmoid_kernel, dim3((1u << 24) / 256), dim3(256), 0, nullptr, hi, d + 1);
+        n += 3ull << 24;
+    }
     if (random_pairs) {
         const unsigned long long chunk = 1ull << 26;
+        for (unsigned long long off = 0; off < random_pairs && e == hipSuccess; off += chunk) {
+            const unsigned long long m = random_pairs - off < chunk ? random_pairs - off : chunk;
+            hipLaunchKernelGGL(div_selftest_refraction_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
+                               nullptr, 0x5851F42D4C957F2Dull + off, m, d + 1);
+            n += 3 * m;
+        }
         for (unsigned long long off = 0; off < random_pairs && e == hipSuccess; off += chunk) {
             const unsigned long long m = random_pairs - off < chunk ? random_pairs - off : chunk;
             hipLaunchKernelGGL(div_selftest_random_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, nullptr,

@@ -797,7 +797,7 @@ std::string user_field_source(const std::string& translated) {
       << "    }\n"
          "};\n"
          "}  // namespace lw\n"
-         "extern \"C\" __global__ void __launch_bounds__(64) linon_raycast_user(RaycastParams P) {\n"
+         "extern \"C\" __global__ void __launch_bounds__(64) LK_RAYCAST_OCCUPANCY linon_raycast_user(RaycastParams P) {\n"
          "    lk::raycast_body<lw::UserField>(P);\n"
          "}\n";
     return o.str();
