@@ -481,15 +481,26 @@ __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, co
         r.d = rn::add(rn::mul(rn::sub(1.0f, w), r.d), rn::mul(w, V));
         const vec3 step = rn::mul(r.d, h);
         if (r.face < 0) {
-            const float L = rn::length(step);
-            bool query = (L > 0.0f && L < __builtin_inff());  // normalize(0) or NaN: no hit
+            const float L2 = rn::dot(step, step);
+            // normalize(0) or NaN: no hit (L2 > 0 and finite exactly when |step| rounds so)
+            bool query = (L2 > 0.0f && L2 < __builtin_inff());
             if (query && !brute) {
-                if (!(r.safe > L + margin)) r.safe = grid_safe_distance(S, p);
-                query = !(r.safe > L + margin);
+                // The skip test and the distance bound need only an upper bound of the segment
+                // length.  The IEEE build takes it from the hardware square root (within a few
+                // ulp) scaled up by 2^-20, which is at least the correctly rounded length;
+                // the exact length is formed only for a query.
+#ifdef LINON_FAST_MATH
+                const float L_hi = rn::length(step);
+#else
+                const float L_hi = __builtin_amdgcn_sqrtf(L2) * (1.0f + 0x1p-20f);
+#endif
+                if (!(r.safe > L_hi + margin)) r.safe = grid_safe_distance(S, p);
+                query = !(r.safe > L_hi + margin);
                 st.skips += query ? 0u : 1u;
-                r.safe -= L;
+                r.safe -= L_hi;
             }
             if (query) {
+                const float L = rn::length(step);
                 st.queries += 1;
                 const vec3 nd = step / L;  // normalize(step_dir)
                 float best_t = -1.0f;
