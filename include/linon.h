@@ -206,6 +206,10 @@ linon_status linon_resize(linon_context* ctx, uint32_t width, uint32_t height);
 linon_status linon_set_camera(linon_context* ctx, const linon_camera* camera);
 /* MainView::update_settings (src/main_view.rs:547-550). */
 linon_status linon_set_settings(linon_context* ctx, const linon_settings* settings);
+/* The overlay pass's entry point: 0 = overlay_desktop (the default), 1 = overlay_web, which
+ * stores the sRGB encoding of the final colour's rgb (src/overlay.wgsl:148-165); the reference
+ * picks overlay_web for its wasm32 build (src/main_view.rs:422-426). */
+linon_status linon_set_output_srgb(linon_context* ctx, int enable);
 /* MainView::update_model (src/main_view.rs:460-488): the vertex/face storage buffers of
  * bind group 1.  Copied to the device; the acceleration structure is built here.  The face
  * index (position in `faces`) is the hit id and the tie-break key. */

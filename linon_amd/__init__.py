@@ -234,6 +234,7 @@ SIGNATURES = {
     "linon_selftest_division": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                           C.POINTER(C.c_uint64)]),
     "linon_read_grid": (C.c_int, [_P, _P, _P, C.c_size_t, _P, C.c_size_t, _P, C.c_size_t, _P, C.c_size_t, _P]),
+    "linon_set_output_srgb": (C.c_int, [_P, C.c_int]),
     "linon_set_field_predefined": (C.c_int, [_P, C.c_int]),
     "linon_set_field_wgsl": (C.c_int, [_P, C.c_char_p]),
     "linon_wgsl_translate": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
@@ -415,6 +416,10 @@ class MainView:
         out["cell_faces"] = out["cell_faces"][:refs.value]
         out["tris"] = out["tris"][:self._num_faces]
         return out
+
+    def set_output_srgb(self, enable: bool) -> None:
+        """overlay_web (sRGB-encoded final colour) instead of overlay_desktop."""
+        self._check(self._lib.linon_set_output_srgb(self._h, 1 if enable else 0))
 
     def set_field(self, field) -> None:
         fid = FIELDS[field] if isinstance(field, str) else int(field)

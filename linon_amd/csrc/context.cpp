@@ -406,6 +406,14 @@ linon_status linon_set_camera(linon_context* ctx, const linon_camera* camera) {
     return LINON_OK;
 }
 
+linon_status linon_set_output_srgb(linon_context* ctx, int enable) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    LN_CTX(c);
+    if (enable != 0 && enable != 1) return c->fail(LINON_ERR_ARGUMENT, "set_output_srgb: enable must be 0 or 1");
+    c->srgb_output = enable == 1;
+    return LINON_OK;
+}
+
 linon_status linon_set_settings(linon_context* ctx, const linon_settings* s) {
     Context* c = reinterpret_cast<Context*>(ctx);
     LN_CTX(c);

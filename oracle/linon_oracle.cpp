@@ -558,9 +558,20 @@ T lyapunov(V3<T> x_next, V3<T> x_prev, V3<T> y_next, V3<T> y_prev) {
     return std::sqrt(std::max(e0, e1));
 }
 
+// srgb_from_linear, overlay.wgsl:148-154: per channel, x / 12.92 below 0.0031308, else
+// 1.055 x^(1/2.4) - 0.055 (1.0 / 2.4 is an f32 constant expression in WGSL).
+template <typename T>
+T srgb_from_linear(T x) {
+    const T inv = T(1.0f / 2.4f);
+    return x < T(0.0031308f) ? x / T(12.92f) : T(1.055f) * std::pow(x, inv) - T(0.055f);
+}
+
+// overlay_desktop (overlay.wgsl:137-145) and, with web, overlay_web (:156-165), which stores the
+// sRGB encoding of the colour's rgb (alpha unchanged); the reference picks overlay_web for its
+// wasm32 build (main_view.rs:422-426).
 template <typename T>
 int overlay(const SettingsIn* st, int W, int H, const uint8_t* ray_casting, const T* mapping,
-            uint8_t* final8, float* exponents, float* lyap) {
+            uint8_t* final8, float* exponents, float* lyap, bool web = false) {
     if (!st || W < 1 || H < 1) return -1;
     const int delta = st->central_difference_delta;
     const int padding = delta * 2;
@@ -587,6 +598,8 @@ int overlay(const SettingsIn* st, int W, int H, const uint8_t* ray_casting, cons
                     for (int c = 0; c < 4; ++c) out[c] = alpha * T(1) + (T(1) - alpha) * color[c];
                 }
             }
+            if (web)
+                for (int c = 0; c < 3; ++c) out[c] = srgb_from_linear(out[c]);
             for (int c = 0; c < 4; ++c) final8[4 * idx + c] = unorm8(double(out[c]));
         }
     }
@@ -754,7 +767,7 @@ void flowmap(int field, const IntegratorIn& ig, T field_weight, const T* in, T* 
 
 extern "C" {
 
-int oracle_abi_version() { return 6; }
+int oracle_abi_version() { return 7; }
 void oracle_set_nudge(int ulps) { g_nudge = ulps > 0 ? 1 : (ulps < 0 ? -1 : 0); }
 void oracle_set_prefilter(int on) { g_prefilter = on != 0; }
 
@@ -794,6 +807,11 @@ int oracle_overlay_f64(const SettingsIn* st, int W, int H, const uint8_t* ray_ca
                        uint8_t* final8, float* exponents, float* lyap) {
     return overlay<double>(st, W, H, ray_casting, mapping, final8, exponents, lyap);
 }
+int oracle_overlay_web_f32(const SettingsIn* st, int W, int H, const uint8_t* ray_casting, const float* mapping,
+                           uint8_t* final8, float* exponents, float* lyap) {
+    return overlay<float>(st, W, H, ray_casting, mapping, final8, exponents, lyap, true);
+}
+float oracle_srgb_from_linear_f32(float x) { return srgb_from_linear<float>(x); }
 int oracle_sample_rays_f32(const CameraIn* cam, const SettingsIn* st, const IntegratorIn* ig, int field, int W,
                            int H, float* out) {
     return sample_rays<float>(cam, st, ig, field, W, H, out);

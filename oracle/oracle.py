@@ -101,6 +101,9 @@ def _bind(_lib: C.CDLL) -> C.CDLL:
     _lib.oracle_render_pixels_f64.argtypes = _lib.oracle_render_pixels_f32.argtypes
     _lib.oracle_overlay_f32.argtypes = [P, C.c_int, C.c_int, P, P, P, P, P]
     _lib.oracle_overlay_f64.argtypes = _lib.oracle_overlay_f32.argtypes
+    _lib.oracle_overlay_web_f32.argtypes = _lib.oracle_overlay_f32.argtypes
+    _lib.oracle_srgb_from_linear_f32.argtypes = [C.c_float]
+    _lib.oracle_srgb_from_linear_f32.restype = C.c_float
     _lib.oracle_sample_rays_f32.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int, P]
     _lib.oracle_sample_rays_f64.argtypes = _lib.oracle_sample_rays_f32.argtypes
     _lib.oracle_outline_search.argtypes = [P, C.c_int, C.c_int, P]
@@ -258,8 +261,16 @@ def threads_available() -> int:
     return int(lib().oracle_max_threads())
 
 
-def overlay(settings, ray_casting8, mapping, precision: str = "f32") -> dict:
+def srgb_from_linear(x: float) -> float:
+    """overlay.wgsl:148-154 in f32 (one channel)."""
+    return float(lib().oracle_srgb_from_linear_f32(float(x)))
+
+
+def overlay(settings, ray_casting8, mapping, precision: str = "f32", web: bool = False) -> dict:
+    """overlay_desktop, or with web=True overlay_web (sRGB-encoded rgb; f32 only)."""
     L = lib()
+    if web and precision != "f32":
+        raise ValueError("overlay_web is restated in f32 only")
     H, W = ray_casting8.shape[:2]
     rc8 = np.ascontiguousarray(ray_casting8, dtype=np.uint8)
     ft = np.float32 if precision == "f32" else np.float64
@@ -267,7 +278,7 @@ def overlay(settings, ray_casting8, mapping, precision: str = "f32") -> dict:
     final = np.zeros((H, W, 4), np.uint8)
     expo = np.zeros(W * H, np.float32)
     lyap = np.zeros((H, W), np.float32)
-    fn = L.oracle_overlay_f32 if precision == "f32" else L.oracle_overlay_f64
+    fn = (L.oracle_overlay_web_f32 if web else L.oracle_overlay_f32) if precision == "f32" else L.oracle_overlay_f64
     rc = fn(C.byref(settings), W, H, _p(rc8), _p(m), _p(final), _p(expo), _p(lyap))
     if rc != 0:
         raise ValueError(f"oracle_overlay failed: {rc}")

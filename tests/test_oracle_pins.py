@@ -453,3 +453,55 @@ def test_outline_search_row_range_quirks():
     W, H = 3, 6
     got = O.outline_search(np.ones(W * H, np.float32), W, H)
     assert tuple(got[0]) == (0, 2, 1)
+
+
+# ---- overlay_web's sRGB encoding (overlay.wgsl:148-165) ----------------------------------------
+
+def _srgb_decode(c):
+    """The sRGB decoding function of IEC 61966-2-1 (the inverse of the encoding)."""
+    c = np.asarray(c, np.float64)
+    return np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4)
+
+
+def test_srgb_encoding_known_values_and_inverse():
+    """overlay.wgsl:148-154.  Above the 0.0031308 cutoff it is the sRGB encoding of IEC 61966-2-1:
+    its published values (linear 0.5 -> 0.7353570, 0.18 -> 0.4613561, 1 -> 1) and a round trip
+    through the standard's decoding, which a wrong exponent or branch would break.  Below the
+    cutoff the shader divides by 12.92 where the standard multiplies: kept as the reference
+    writes it (0.001 -> 0.001 / 12.92), so the function jumps at the cutoff."""
+    assert O.srgb_from_linear(0.0) == 0.0
+    assert abs(O.srgb_from_linear(1.0) - 1.0) <= 1e-6
+    assert abs(O.srgb_from_linear(0.5) - 0.7353569830524495) <= 2e-6
+    assert abs(O.srgb_from_linear(0.18) - 0.46135612950044164) <= 2e-6
+    assert O.srgb_from_linear(0.001) == float(np.float32(0.001) / np.float32(12.92))
+    below = O.srgb_from_linear(np.nextafter(np.float32(0.0031308), np.float32(0)))
+    assert abs(below - 0.0031308 / 12.92) <= 1e-8
+    assert abs(O.srgb_from_linear(0.0031308) - 0.040448) <= 1e-5
+    xs = np.linspace(0.0031308, 1.0, 1001)
+    enc = np.array([O.srgb_from_linear(x) for x in xs])
+    np.testing.assert_allclose(_srgb_decode(enc), xs, rtol=2e-5, atol=1e-7)
+    assert (np.diff(enc) >= 0).all()
+
+
+def test_overlay_web_encodes_the_copied_colour():
+    """overlay_mode 0 with overlay_web: every texel's rgb is the sRGB encoding of its colour,
+    alpha untouched (against the published decoding: within one unorm8 step)."""
+    W, H = 16, 16
+    rc = np.zeros((H, W, 4), np.uint8)
+    rc[..., 0] = np.arange(256, dtype=np.uint8).reshape(H, W)
+    rc[..., 1] = 255 - rc[..., 0]
+    rc[..., 2] = 7
+    rc[..., 3] = np.arange(256, dtype=np.uint8).reshape(H, W)[::-1]
+    mapping = np.zeros((H, W, 4), np.float32)
+    st = O.settings_in(overlay_mode=0)
+    web = O.overlay(st, rc, mapping, web=True)["final"]
+    desk = O.overlay(st, rc, mapping)["final"]
+    assert np.array_equal(desk, rc)
+    assert np.array_equal(web[..., 3], rc[..., 3])
+    # decoding the encoded byte lands within one (encoded) step of the original linear value
+    enc = web[..., :3].astype(int)
+    hi = _srgb_decode(np.minimum(enc + 1, 255) / 255.0)
+    lo = _srgb_decode(np.maximum(enc - 1, 0) / 255.0)
+    x = rc[..., :3] / 255.0
+    assert ((lo <= x + 1e-9) & (x <= hi + 1e-9)).all()
+    assert (enc >= rc[..., :3]).all()
