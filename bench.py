@@ -17,6 +17,7 @@ the launch stream over the timed region.  `cpu_baseline` times the FP32 CPU rest
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -211,6 +212,10 @@ def main() -> int:
     if not a.skip_stress:
         log("4K stress ...")
         extra["stress_4k"] = bench_stress()
+        if not a.skip_surrogate:
+            log("4K stress surrogate ...")
+            extra["stress_4k"]["surrogate"] = bench_surrogate(a.surrogate_steps, configs.CONFIG5.fields,
+                                                              configs.CONFIG5)
     out["other_workloads"] = extra
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
@@ -221,10 +226,46 @@ def main() -> int:
     return 0
 
 
+def one_graph_ms(view, fields) -> float:
+    """Captures one render per field on a new stream into a single HIP graph (through the HIP
+    runtime the library already runs on), replays it once to warm up and returns the device time
+    of one more replay."""
+    hip = C.CDLL("libamdhip64.so.7")
+    vp = C.c_void_p
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed: {rc}")
+
+    stream, graph, exe, e0, e1 = vp(), vp(), vp(), vp(), vp()
+    ok(hip.hipStreamCreate(C.byref(stream)), "hipStreamCreate")
+    ok(hip.hipEventCreate(C.byref(e0)), "hipEventCreate")
+    ok(hip.hipEventCreate(C.byref(e1)), "hipEventCreate")
+    ok(hip.hipStreamBeginCapture(stream, 1), "hipStreamBeginCapture")  # thread-local mode
+    for name in fields:
+        view.set_field(name)
+        view.render(stream=stream.value)
+    ok(hip.hipStreamEndCapture(stream, C.byref(graph)), "hipStreamEndCapture")
+    ok(hip.hipGraphInstantiateWithFlags(C.byref(exe), graph, C.c_ulonglong(0)), "hipGraphInstantiateWithFlags")
+    ok(hip.hipGraphLaunch(exe, stream), "hipGraphLaunch")
+    ok(hip.hipEventRecord(e0, stream), "hipEventRecord")
+    ok(hip.hipGraphLaunch(exe, stream), "hipGraphLaunch")
+    ok(hip.hipEventRecord(e1, stream), "hipEventRecord")
+    ok(hip.hipStreamSynchronize(stream), "hipStreamSynchronize")
+    ms = C.c_float(0.0)
+    ok(hip.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+    hip.hipGraphExecDestroy(exe)
+    hip.hipGraphDestroy(graph)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipStreamDestroy(stream)
+    return float(ms.value)
+
+
 def bench_stress() -> dict:
     """3840x2160 at h = 0.001 (5001 RK4 iterations per ray) against the Cornell box plus the
     1,310,720-face level-8 icosphere, lens and hot-air, overlay on, 8 sample rays x 64 records:
-    one warm-up and one timed frame per field."""
+    one warm-up and one timed frame per field, then both frames in one captured HIP graph."""
     wl = configs.CONFIG5
     verts, faces = wl.mesh()
     view = L.MainView(wl.width, wl.height)
@@ -249,6 +290,10 @@ def bench_stress() -> dict:
                                   "gsteps_per_s": c["rk4_steps"] / (r_ms * 1e-3) / 1e9,
                                   "tflops": fl / (r_ms * 1e-3) / 1e12}
         log(f"stress {name}: {rms[0]:.1f} ms")
+    # both fields' frames captured into one HIP graph on a caller's stream and replayed (the
+    # graph keeps each capture's field; the frames share the output buffers)
+    out["one_graph_all_fields_ms"] = one_graph_ms(view, wl.fields)
+    log(f"stress one graph ({'+'.join(wl.fields)}): {out['one_graph_all_fields_ms']:.1f} ms")
     view.close()
     return out
 
@@ -299,12 +344,11 @@ def bench_batch(verts, faces, replays: int) -> dict:
             "tuning": tuning.as_dict(), "autotune_full_size_ms": tuned_ms, "autotune_seconds": tune_s}
 
 
-def bench_surrogate(steps: int, fields) -> dict:
+def bench_surrogate(steps: int, fields, wl=configs.CONFIG2) -> dict:
     """Per field: train the flow-map surrogate (2^24 uniform entry states labelled by the FP32
-    RK4 kernel, batch 65536, Adam lr 1e-3), then render the 1080p view of the benchmark camera
+    RK4 kernel, batch 65536, Adam lr 1e-3), then render the workload's view (1080p by default)
     without the mesh (mapping + overlay) twice: plain RK4 (escape mode) and surrogate +
     fallback; report speed-up and the exit-point error of every pixel against the plain frame."""
-    wl = configs.CONFIG2
     out = {}
     for field in fields:
         view = L.MainView(wl.width, wl.height)
@@ -343,7 +387,8 @@ def bench_surrogate(steps: int, fields) -> dict:
                       "mlp_tflops": (int(st["entered"]) * 498176 / (st["mlp_ms"] * 1e-3) / 1e12
                                      if st["mlp_ms"] > 0 else None)}
         view.close()
-        log(f"surrogate {field}: {out[field]['speedup']}x, within 1e-3: {out[field]['frac_pixels_within_1e-3']:.4f}")
+        log(f"surrogate {field} {wl.width}x{wl.height}: {out[field]['speedup']}x, "
+            f"within 1e-3: {out[field]['frac_pixels_within_1e-3']:.4f}")
     return out
 
 
