@@ -50,6 +50,7 @@ def test_overlay_web_srgb_epilogue(view):
     mid = (desk["final"][..., :3] > 0) & (desk["final"][..., :3] < 255)
     assert mid.any() and (gpu.final[..., :3][mid] >= desk["final"][..., :3][mid]).all()
     assert np.array_equal(gpu.final[..., 3], desk["final"][..., 3])
+    assert view._lib.linon_set_output_srgb(view._h, 2) == 1  # LINON_ERR_ARGUMENT
 
 
 def test_device_buffers_are_the_outputs(view):
