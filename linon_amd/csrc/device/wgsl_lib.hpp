@@ -87,7 +87,24 @@ __device__ __forceinline__ vec3 cross(vec3 a, vec3 b) {
 }
 __device__ __forceinline__ float length(vec3 a) { return __builtin_sqrtf(dot(a, a)); }
 __device__ __forceinline__ float length(float a) { return __builtin_fabsf(a); }
-__device__ __forceinline__ vec3 normalize(vec3 a) { return a / length(a); }
+// v / s for callers whose components are bounded through the denominator: every |v_i| is at
+// most 2^61 whenever 2^-60 <= s <= 2^60 (normalize: |v_i| <= length(v) to an ulp; the lens
+// field: |0.3 p_i| <= 0.3 r <= 0.3 s^(1/3) with s = max(r^3, 0.001)).  The IEEE build's range
+// test then needs only s and the smallest component magnitude; NaN or infinite components make
+// s NaN or infinite there, which fails the test as in div3_rn.  Same result as v / s.
+__device__ __forceinline__ vec3 div3_rn_bounded(vec3 v, float s) {
+#ifdef LINON_FAST_MATH
+    return v / s;
+#else
+    const float lo = __builtin_fminf(__builtin_fminf(__builtin_fabsf(v.x), __builtin_fabsf(v.y)), __builtin_fabsf(v.z));
+    if (s >= 0x1p-60f && s <= 0x1p60f && lo >= 0x1p-60f) {
+        const float y = recip_rn(s);
+        return {quot_rn(v.x, s, y), quot_rn(v.y, s, y), quot_rn(v.z, s, y)};
+    }
+    return {v.x / s, v.y / s, v.z / s};
+#endif
+}
+__device__ __forceinline__ vec3 normalize(vec3 a) { return div3_rn_bounded(a, length(a)); }
 __device__ __forceinline__ float distance(vec3 a, vec3 b) { return length(a - b); }
 __device__ __forceinline__ vec3 reflect(vec3 e1, vec3 e2) { return e1 - (2.0f * dot(e2, e1)) * e2; }
 

@@ -186,6 +186,20 @@ __global__ void __launch_bounds__(256) div_selftest_random_kernel(unsigned long 
     bad += fbits(q.y) != fbits(c / b);
     bad += fbits(q.z) != fbits(-a / b);
     bad += fbits(lw::div_rn(a, b)) != fbits(a / b);
+    // normalize with its shortened range test against the component-wise IEEE quotients
+    const lw::vec3 v = lw::vec3f(a, b, c);
+    const float l = lw::length(v);
+    const lw::vec3 u = lw::normalize(v);
+    bad += !same_or_nan(u.x, a / l);
+    bad += !same_or_nan(u.y, b / l);
+    bad += !same_or_nan(u.z, c / l);
+    // the lens field's quotient 0.3 v / max(|v|^3, 0.001)
+    const lw::vec3 w = 0.3f * v;
+    const float m = __builtin_fmaxf(l * l * l, 0.001f);
+    const lw::vec3 ql = lw::div3_rn_bounded(w, m);
+    bad += !same_or_nan(ql.x, w.x / m);
+    bad += !same_or_nan(ql.y, w.y / m);
+    bad += !same_or_nan(ql.z, w.z / m);
     if (bad) atomicAdd(mism, (unsigned long long)bad);
 }
 
@@ -253,7 +267,7 @@ hipError_t run_division_selftest(unsigned long long random_pairs, unsigned long 
             const unsigned long long m = random_pairs - off < chunk ? random_pairs - off : chunk;
             hipLaunchKernelGGL(div_selftest_random_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, nullptr,
                                0xD1B54A32D192ED03ull + off, m, d + 1);
-            n += 4 * m;
+            n += 10 * m;
         }
     }
     if (e == hipSuccess) e = hipGetLastError();
