@@ -111,9 +111,9 @@ def main() -> int:
     ap.add_argument("--skip-stress", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the ray-cast pass has no multi-GPU path in this build "
-              "(rays are independent; sharding by image tiles or views is described in DESIGN.md)",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the ray-cast pass has no multi-GPU path "
+              "(rays are independent: it would shard by image tiles or by views, exchanging only "
+              "the overlay's border rows between tiles)", file=sys.stderr)
         return 2
 
     wl = configs.CONFIG2
