@@ -37,10 +37,11 @@ def _render_both(view, W, H, cam, st, ig, field, verts, faces):
     return g, r
 
 
-@pytest.mark.parametrize("W,H", [(2, 2), (9, 7), (67, 33), (8, 70)])
+@pytest.mark.parametrize("W,H", [(2, 2), (9, 7), (67, 33), (8, 70), (16384, 2), (2, 16384)])
 def test_ragged_sizes_identity_field(view, W, H):
     """The identity field is a straight line: no field arithmetic, so hit faces must agree
-    exactly and end points to rounding, on images whose edge tiles are partial."""
+    exactly and end points to rounding, on images whose edge tiles are partial, up to the
+    largest width and height the context accepts (16384)."""
     wl = configs.CONFIG1
     v, f = wl.mesh()
     ig = L.Integrator.make(h_initial=0.01, adaptive=False, max_steps=1000, samples_per_ray=0, sample_steps=0)
