@@ -29,34 +29,82 @@ __device__ __forceinline__ void load_face(const float* xyz, const uint32_t* face
     }
 }
 
-// Count (FILL = false) or place (FILL = true) the references of one face per lane.
+// Faces whose cell range holds more than this many cells are tested cell-parallel by a whole
+// block (grid_cells_big_kernel) instead of by one lane: a wall of the Cornell box spans 2 x 256^2
+// cells of the 256^3 grid, against a handful for an icosphere face.
+constexpr size_t kBigFaceCells = 256;
+
+__device__ __forceinline__ size_t range_cells(const int c0[3], const int c1[3]) {
+    return (size_t)(c1[0] - c0[0] + 1) * (size_t)(c1[1] - c0[1] + 1) * (size_t)(c1[2] - c0[2] + 1);
+}
+
+// One reference of face f to cell (x, y, z): count it (FILL = false) or place it (FILL = true).
+template <bool FILL>
+__device__ __forceinline__ void add_reference(const GridGeom& g, size_t f, int x, int y, int z, unsigned* counter,
+                                              const unsigned* cell_start, unsigned* cell_faces, unsigned* overflow) {
+    const size_t cell = ((size_t)z * (size_t)g.dim[1] + (size_t)y) * (size_t)g.dim[0] + (size_t)x;
+    if (!FILL) {
+        atomicAdd(counter + cell, 1u);
+    } else {
+        const unsigned slot = atomicAdd(counter + cell, 1u);
+        const unsigned b = cell_start[cell], e = cell_start[cell + 1];
+        if (slot < e - b)
+            cell_faces[b + slot] = (unsigned)f;
+        else
+            atomicOr(overflow, 1u);  // counts and placements disagree: reported, never written
+    }
+}
+
+// Count (FILL = false) or place (FILL = true) the references of one face per lane.  Faces with
+// more than kBigFaceCells cells in range are left to grid_cells_big_kernel; the counting pass
+// lists them in big[0 .. *nbig).
 template <bool FILL>
 __global__ void __launch_bounds__(256) grid_cells_kernel(GridGeom g, const float* xyz, const uint32_t* faces,
                                                          uint32_t nf, unsigned* counter, const unsigned* cell_start,
-                                                         unsigned* cell_faces, unsigned* overflow) {
+                                                         unsigned* cell_faces, unsigned* overflow, unsigned* big,
+                                                         unsigned* nbig) {
     const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nf) return;
     double v[3][3];
     load_face(xyz, faces, (uint32_t)f, v);
     int c0[3], c1[3];
     face_cell_range(g, v, c0, c1);
-    const size_t nx = (size_t)g.dim[0], ny = (size_t)g.dim[1];
+    if (c1[0] < c0[0] || c1[1] < c0[1] || c1[2] < c0[2]) return;  // outside the grid (cannot happen)
+    if (range_cells(c0, c1) > kBigFaceCells) {
+        if (!FILL) big[atomicAdd(nbig, 1u)] = (unsigned)f;  // at most nf entries
+        return;
+    }
     for (int z = c0[2]; z <= c1[2]; ++z)
         for (int y = c0[1]; y <= c1[1]; ++y)
-            for (int x = c0[0]; x <= c1[0]; ++x) {
-                if (!face_in_cell(g, v, x, y, z)) continue;
-                const size_t cell = ((size_t)z * ny + (size_t)y) * nx + (size_t)x;
-                if (!FILL) {
-                    atomicAdd(counter + cell, 1u);
-                } else {
-                    const unsigned slot = atomicAdd(counter + cell, 1u);
-                    const unsigned b = cell_start[cell], e = cell_start[cell + 1];
-                    if (slot < e - b)
-                        cell_faces[b + slot] = (unsigned)f;
-                    else
-                        atomicOr(overflow, 1u);  // counts and placements disagree: reported, never written
-                }
-            }
+            for (int x = c0[0]; x <= c1[0]; ++x)
+                if (face_in_cell(g, v, x, y, z))
+                    add_reference<FILL>(g, f, x, y, z, counter, cell_start, cell_faces, overflow);
+}
+
+// The faces listed by the counting pass, one block per face at a time (grid-stride over the
+// list), its lanes striding over the face's cell range.
+template <bool FILL>
+__global__ void __launch_bounds__(256) grid_cells_big_kernel(GridGeom g, const float* xyz, const uint32_t* faces,
+                                                             const unsigned* big, const unsigned* nbig,
+                                                             unsigned* counter, const unsigned* cell_start,
+                                                             unsigned* cell_faces, unsigned* overflow) {
+    const unsigned n = *nbig;
+    for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {
+        const size_t f = big[i];
+        double v[3][3];
+        load_face(xyz, faces, (uint32_t)f, v);
+        int c0[3], c1[3];
+        face_cell_range(g, v, c0, c1);
+        const size_t sx = (size_t)(c1[0] - c0[0] + 1), sy = (size_t)(c1[1] - c0[1] + 1);
+        const size_t cells = range_cells(c0, c1);
+        for (size_t k = threadIdx.x; k < cells; k += blockDim.x) {
+            const int x = c0[0] + (int)(k % sx);
+            const int y = c0[1] + (int)((k / sx) % sy);
+            const int z = c0[2] + (int)(k / (sx * sy));
+            if (face_in_cell(g, v, x, y, z))
+                add_reference<FILL>(g, f, x, y, z, counter, cell_start, cell_faces, overflow);
+        }
+    }
 }
 
 __device__ __forceinline__ unsigned long long block_inclusive_scan(unsigned long long v, unsigned long long* s) {
@@ -226,6 +274,8 @@ bool build_grid_device(const GridGeom& g, const float* h_xyz, uint32_t nv, const
     unsigned* d_count = nullptr;
     unsigned long long *d_local = nullptr, *d_bsum = nullptr, *d_total = nullptr;
     unsigned* d_flag = nullptr;
+    unsigned* d_big = nullptr;   // faces left to the block-per-face kernel, and their count
+    unsigned* d_nbig = nullptr;
     unsigned char* d_tmp = nullptr;
     bool ok = false;
     auto check = [&](hipError_t e, const char* what) {
@@ -238,6 +288,8 @@ bool build_grid_device(const GridGeom& g, const float* h_xyz, uint32_t nv, const
             !check(hipMalloc(&d_count, nc * 4), "alloc") || !check(hipMalloc(&d_local, nc * 8), "alloc") ||
             !check(hipMalloc(&d_bsum, nb * 8), "alloc") || !check(hipMalloc(&d_total, 8), "alloc") ||
             !check(hipMalloc(&d_flag, 4), "alloc") || !check(hipMalloc(&d_tmp, nc), "alloc") ||
+            !check(hipMalloc(&d_big, std::max<size_t>(1, (size_t)nf) * 4), "alloc") ||
+            !check(hipMalloc(&d_nbig, 4), "alloc") ||
             !check(hipMalloc(&out.cell_start, (nc + 1) * 4), "alloc") || !check(hipMalloc(&out.dist, nc), "alloc") ||
             !check(hipMalloc(&out.tris, std::max<size_t>(1, (size_t)nf) * 48), "alloc"))
             break;
@@ -246,9 +298,15 @@ bool build_grid_device(const GridGeom& g, const float* h_xyz, uint32_t nv, const
             break;
         hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(nc, 256)), dim3(256), 0, s, d_count, nc, 0u);
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, s, d_flag, (size_t)1, 0u);
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, s, d_nbig, (size_t)1, 0u);
+        const unsigned big_blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(nf, 8192));
         if (nf) {
             hipLaunchKernelGGL(grid_cells_kernel<false>, dim3(blocks_for(nf, 256)), dim3(256), 0, s, g, d_xyz,
-                               d_faces, nf, d_count, (const unsigned*)nullptr, (unsigned*)nullptr, d_flag);
+                               d_faces, nf, d_count, (const unsigned*)nullptr, (unsigned*)nullptr, d_flag, d_big,
+                               d_nbig);
+            hipLaunchKernelGGL(grid_cells_big_kernel<false>, dim3(big_blocks), dim3(256), 0, s, g, d_xyz, d_faces,
+                               (const unsigned*)d_big, (const unsigned*)d_nbig, d_count, (const unsigned*)nullptr,
+                               (unsigned*)nullptr, d_flag);
             hipLaunchKernelGGL(pack_faces_kernel, dim3(blocks_for(nf, 256)), dim3(256), 0, s, d_xyz, d_faces, nf,
                                reinterpret_cast<float4*>(out.tris));
         }
@@ -270,9 +328,14 @@ bool build_grid_device(const GridGeom& g, const float* h_xyz, uint32_t nv, const
         if (!check(hipMalloc(&out.cell_faces, std::max<size_t>(1, (size_t)total) * 4), "alloc")) break;
         out.refs = (size_t)total;
         hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(nc, 256)), dim3(256), 0, s, d_count, nc, 0u);
-        if (nf)
+        if (nf) {
             hipLaunchKernelGGL(grid_cells_kernel<true>, dim3(blocks_for(nf, 256)), dim3(256), 0, s, g, d_xyz,
-                               d_faces, nf, d_count, (const unsigned*)out.cell_start, out.cell_faces, d_flag);
+                               d_faces, nf, d_count, (const unsigned*)out.cell_start, out.cell_faces, d_flag, d_big,
+                               d_nbig);
+            hipLaunchKernelGGL(grid_cells_big_kernel<true>, dim3(big_blocks), dim3(256), 0, s, g, d_xyz, d_faces,
+                               (const unsigned*)d_big, (const unsigned*)d_nbig, d_count,
+                               (const unsigned*)out.cell_start, out.cell_faces, d_flag);
+        }
         hipLaunchKernelGGL(sort_cells_kernel, dim3(blocks_for(nc, 256)), dim3(256), 0, s,
                            (const unsigned*)out.cell_start, nc, out.cell_faces);
         hipLaunchKernelGGL(dist_x_kernel, dim3(blocks_for(ny * nz, 256)), dim3(256), 0, s,
@@ -293,7 +356,7 @@ bool build_grid_device(const GridGeom& g, const float* h_xyz, uint32_t nv, const
         ok = true;
     } while (false);
     for (void* p : {(void*)d_xyz, (void*)d_faces, (void*)d_count, (void*)d_local, (void*)d_bsum, (void*)d_total,
-                    (void*)d_flag, (void*)d_tmp})
+                    (void*)d_flag, (void*)d_tmp, (void*)d_big, (void*)d_nbig})
         if (p) (void)hipFree(p);
     if (!ok) out.release();
     return ok;

@@ -16,7 +16,7 @@ def _grid(view, builder):
     return view.read_grid()
 
 
-@pytest.mark.parametrize("mesh,res", [("cornell", 0), ("ico3", 0), ("ico6", 128), ("empty", 0)])
+@pytest.mark.parametrize("mesh,res", [("cornell", 0), ("ico3", 0), ("ico6", 128), ("ico6", 256), ("empty", 0)])
 def test_device_grid_equals_host_grid(mesh, res):
     if mesh == "cornell":
         v, f = scenes.cornell_box_centered()
