@@ -170,6 +170,10 @@ def main() -> int:
                 "flop_per_launch": d["flop"], "launch_ms": d["raycast_ms"],
                 "all_raycast_kernels_tflops": sum(p["flop"] for p in per_field.values())
                 / (sum(p["raycast_ms"] for p in per_field.values()) * 1e-3) / 1e12}
+    # the attainable rate without packed FP32, measured on this device
+    fma = L.measure_fma_rate()
+    roofline["measured_fp32_fma_tflops"] = fma
+    roofline["frac_of_measured_fma"] = d["tflops"] / fma if fma > 0 else None
     traffic_file = ROOT / "profiles" / "traffic.json"
     if traffic_file.exists():
         try:

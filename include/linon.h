@@ -316,6 +316,10 @@ linon_status linon_get_surrogate_stats(linon_context* ctx, linon_surrogate_stats
  * reciprocal's.  Writes the mismatch counts and the number of comparisons. */
 linon_status linon_selftest_division(int device, uint64_t random_pairs, uint64_t* recip_mismatches,
                                      uint64_t* quotient_mismatches, uint64_t* comparisons);
+/* Diagnostics (no counterpart in the reference): the device's measured rate of non-packed FP32
+ * FMA (v_fma_f32, 2 FLOP each), every lane running 16 independent chains over all compute
+ * units; the attainable FP32 rate of kernels built without packed FP32. */
+linon_status linon_measure_fma_rate(int device, double* tflops);
 /* Diagnostics: the surrogate's BF16-MFMA GEMM on host buffers (bf16 bit patterns in, fp32
  * out): C = A * B (+ bias; with silu != 0: C = z = A*B + bias and C_bf16 = bf16(silu(z))).
  * A is [M][K] (a_trans = 0) or [K][M]; B is [K][N] (b_trans = 0) or [N][K]; the K range may

@@ -235,6 +235,7 @@ SIGNATURES = {
                                           C.POINTER(C.c_uint64)]),
     "linon_read_grid": (C.c_int, [_P, _P, _P, C.c_size_t, _P, C.c_size_t, _P, C.c_size_t, _P, C.c_size_t, _P]),
     "linon_set_output_srgb": (C.c_int, [_P, C.c_int]),
+    "linon_measure_fma_rate": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
     "linon_set_field_predefined": (C.c_int, [_P, C.c_int]),
     "linon_set_field_wgsl": (C.c_int, [_P, C.c_char_p]),
     "linon_wgsl_translate": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
@@ -561,6 +562,16 @@ class MainView:
     def read_frame(self) -> Frame:
         return Frame(self.read_final(), self.read_ray_casting(), self.read_mapping(), self.read_hit_faces(),
                      self.read_steps(), self.read_lyapunov(), self.read_exponents(), self.counters())
+
+
+def measure_fma_rate(device: int = 0) -> float:
+    """Measured non-packed FP32 FMA rate of the device in TFLOP/s (v_fma_f32 = 2 FLOP)."""
+    lib = load_library()
+    t = C.c_double(0.0)
+    st = lib.linon_measure_fma_rate(device, C.byref(t))
+    if st != LINON_OK:
+        raise LinonError(st, "linon_measure_fma_rate failed")
+    return float(t.value)
 
 
 def selftest_division(random_pairs: int = 1 << 28, device: int = 0) -> dict:

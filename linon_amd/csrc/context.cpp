@@ -477,6 +477,15 @@ linon_status linon_selftest_division(int device, uint64_t random_pairs, uint64_t
     return LINON_OK;
 }
 
+linon_status linon_measure_fma_rate(int device, double* tflops) {
+    if (!tflops) return LINON_ERR_ARGUMENT;
+    if (hipSetDevice(device) != hipSuccess) return LINON_ERR_HIP;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return LINON_ERR_HIP;
+    if (linon::run_fma_rate(cus, tflops) != hipSuccess) return LINON_ERR_HIP;
+    return LINON_OK;
+}
+
 linon_status linon_set_math(linon_context* ctx, int math) {
     Context* c = reinterpret_cast<Context*>(ctx);
     LN_CTX(c);

@@ -228,6 +228,46 @@ __global__ void __launch_bounds__(256) div_selftest_refraction_kernel(unsigned l
 #endif
 
 #ifndef LINON_FAST_MATH
+// Rate of non-packed FP32 FMA issue: every lane runs 16 independent chains of v_fma_f32 (the
+// build has packed FP32 turned off), so the SIMDs are never short of independent work.
+__global__ void __launch_bounds__(256) fma_rate_kernel(float* out, int iters, float b, float c) {
+    float a[16];
+    for (int k = 0; k < 16; ++k) a[k] = (float)(threadIdx.x + k) * 1e-3f;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) a[k] = __builtin_fmaf(a[k], b, c);
+    }
+    float sum = 0.0f;
+    for (int k = 0; k < 16; ++k) sum += a[k];
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = sum;
+}
+
+hipError_t run_fma_rate(int cus, double* tflops) {
+    const unsigned blocks = (unsigned)(cus > 0 ? cus : 1) * 8u, threads = 256u;
+    const int iters = 2048;
+    float* d = nullptr;
+    hipError_t e = hipMalloc(&d, (size_t)blocks * threads * sizeof(float));
+    if (e != hipSuccess) return e;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float ms = 0.0f;
+    for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {  // the second run is timed
+        e = hipEventRecord(e0, nullptr);
+        if (e == hipSuccess)
+            hipLaunchKernelGGL(fma_rate_kernel, dim3(blocks), dim3(threads), 0, nullptr, d, iters, 0.999f, 0.001f);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(d);
+    if (e == hipSuccess) *tflops = (double)blocks * threads * iters * 16.0 * 2.0 / (ms * 1e-3) / 1e12;
+    return e;
+}
+
 hipError_t run_division_selftest(unsigned long long random_pairs, unsigned long long* recip_mism,
                                  unsigned long long* quot_mism, unsigned long long* cases) {
     unsigned long long* d = nullptr;

@@ -28,6 +28,8 @@ namespace linon {
 LINON_DECLARE_KERNEL_LAUNCHERS
 // IEEE build only: lw::div3_rn / lw::div_rn against the IEEE division, every mantissa of the
 // reciprocal and of numerators against structured denominators, plus random operand pairs
+// Measured rate of non-packed FP32 FMA over `cus` compute units (IEEE build)
+hipError_t run_fma_rate(int cus, double* tflops);
 hipError_t run_division_selftest(unsigned long long random_pairs, unsigned long long* recip_mism,
                                  unsigned long long* quot_mism, unsigned long long* cases);
 }  // namespace linon
