@@ -316,6 +316,10 @@ linon_status linon_get_surrogate_stats(linon_context* ctx, linon_surrogate_stats
  * reciprocal's.  Writes the mismatch counts and the number of comparisons. */
 linon_status linon_selftest_division(int device, uint64_t random_pairs, uint64_t* recip_mismatches,
                                      uint64_t* quotient_mismatches, uint64_t* comparisons);
+/* Diagnostics: the IEEE build's square root (lw::sqrt_rn: the hardware root plus a one-ulp
+ * correction) against the compiler's correctly rounded square root at every one of the 2^32
+ * floats, bit for bit (any NaN matches any NaN). */
+linon_status linon_selftest_sqrt(int device, uint64_t* mismatches, uint64_t* comparisons);
 /* Diagnostics (no counterpart in the reference): the device's measured rate of non-packed FP32
  * FMA (v_fma_f32, 2 FLOP each), every lane running 16 independent chains over all compute
  * units; the attainable FP32 rate of kernels built without packed FP32. */

@@ -236,6 +236,7 @@ SIGNATURES = {
     "linon_read_grid": (C.c_int, [_P, _P, _P, C.c_size_t, _P, C.c_size_t, _P, C.c_size_t, _P, C.c_size_t, _P]),
     "linon_set_output_srgb": (C.c_int, [_P, C.c_int]),
     "linon_measure_fma_rate": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
+    "linon_selftest_sqrt": (C.c_int, [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "linon_set_field_predefined": (C.c_int, [_P, C.c_int]),
     "linon_set_field_wgsl": (C.c_int, [_P, C.c_char_p]),
     "linon_wgsl_translate": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
@@ -562,6 +563,17 @@ class MainView:
     def read_frame(self) -> Frame:
         return Frame(self.read_final(), self.read_ray_casting(), self.read_mapping(), self.read_hit_faces(),
                      self.read_steps(), self.read_lyapunov(), self.read_exponents(), self.counters())
+
+
+def selftest_sqrt(device: int = 0) -> dict:
+    """Bit-for-bit check of the IEEE build's square root against the compiler's correctly
+    rounded one at every float (on the GPU): the mismatch count and the number of comparisons."""
+    lib = load_library()
+    m, n = C.c_uint64(0), C.c_uint64(0)
+    st = lib.linon_selftest_sqrt(device, C.byref(m), C.byref(n))
+    if st != LINON_OK:
+        raise LinonError(st, "linon_selftest_sqrt failed")
+    return {"mismatches": m.value, "comparisons": n.value}
 
 
 def measure_fma_rate(device: int = 0) -> float:

@@ -477,6 +477,16 @@ linon_status linon_selftest_division(int device, uint64_t random_pairs, uint64_t
     return LINON_OK;
 }
 
+linon_status linon_selftest_sqrt(int device, uint64_t* mismatches, uint64_t* comparisons) {
+    if (!mismatches || !comparisons) return LINON_ERR_ARGUMENT;
+    if (hipSetDevice(device) != hipSuccess) return LINON_ERR_HIP;
+    unsigned long long m = 0, n = 0;
+    if (linon::run_sqrt_selftest(&m, &n) != hipSuccess) return LINON_ERR_HIP;
+    *mismatches = m;
+    *comparisons = n;
+    return LINON_OK;
+}
+
 linon_status linon_measure_fma_rate(int device, double* tflops) {
     if (!tflops) return LINON_ERR_ARGUMENT;
     if (hipSetDevice(device) != hipSuccess) return LINON_ERR_HIP;

@@ -83,7 +83,7 @@ __device__ __forceinline__ vec3 cross(vec3 a, vec3 b) {
     return vec3f(sub(mul(a.y, b.z), mul(a.z, b.y)), sub(mul(a.z, b.x), mul(a.x, b.z)),
                  sub(mul(a.x, b.y), mul(a.y, b.x)));
 }
-__device__ __forceinline__ float length(vec3 a) { return __builtin_sqrtf(rn::dot(a, a)); }
+__device__ __forceinline__ float length(vec3 a) { return lw::sqrt_rn(rn::dot(a, a)); }
 __device__ __forceinline__ vec3 normalize(vec3 a) {
     const float l = rn::length(a);
     return vec3f(a.x / l, a.y / l, a.z / l);

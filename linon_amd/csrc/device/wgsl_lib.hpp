@@ -81,11 +81,36 @@ __device__ __forceinline__ vec3 operator+(float s, vec3 a) { return {s + a.x, s 
 __device__ __forceinline__ vec3 operator-(vec3 a, float s) { return {a.x - s, a.y - s, a.z - s}; }
 __device__ __forceinline__ vec3 operator-(float s, vec3 a) { return {s - a.x, s - a.y, s - a.z}; }
 
+// Square root.  The IEEE build forms the correctly rounded root from the hardware one
+// (v_sqrt_f32, within 1 ulp) and a one-ulp correction: of s - ulp, s and s + ulp it keeps the
+// one that the exact (FMA) residuals x - (s -+ ulp) s select.  Inputs below 2^-96, whose
+// residuals could underflow, are scaled by 2^32 first and the root by 2^-16 after.  Zeros,
+// +inf and NaN pass through the correction unchanged, so unlike the compiler's expansion this
+// one has no separate special-case select.  linon_selftest_sqrt compares it with the
+// compiler's square root at every float.
+__device__ __forceinline__ float sqrt_rn(float x) {
+#ifdef LINON_FAST_MATH
+    return __builtin_sqrtf(x);
+#else
+#pragma clang fp contract(off)
+    const bool tiny = x < 0x1p-96f;
+    const float xs = tiny ? x * 0x1p32f : x;
+    const float s = __builtin_amdgcn_sqrtf(xs);
+    const float dn = __uint_as_float(__float_as_uint(s) - 1u);
+    const float up = __uint_as_float(__float_as_uint(s) + 1u);
+    const float r_dn = __builtin_fmaf(-dn, s, xs);
+    const float r_up = __builtin_fmaf(-up, s, xs);
+    float t = r_dn <= 0.0f ? dn : s;
+    t = r_up > 0.0f ? up : t;
+    return tiny ? t * 0x1p-16f : t;
+#endif
+}
+
 __device__ __forceinline__ float dot(vec3 a, vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ vec3 cross(vec3 a, vec3 b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-__device__ __forceinline__ float length(vec3 a) { return __builtin_sqrtf(dot(a, a)); }
+__device__ __forceinline__ float length(vec3 a) { return sqrt_rn(dot(a, a)); }
 __device__ __forceinline__ float length(float a) { return __builtin_fabsf(a); }
 // v / s for callers whose components are bounded through the denominator: every |v_i| is at
 // most 2^61 whenever 2^-60 <= s <= 2^60 (normalize: |v_i| <= length(v) to an ulp; the lens
@@ -125,8 +150,8 @@ LW_UNARY(exp, ::expf(x))
 LW_UNARY(exp2, ::exp2f(x))
 LW_UNARY(log, ::logf(x))
 LW_UNARY(log2, ::log2f(x))
-LW_UNARY(sqrt, __builtin_sqrtf(x))
-LW_UNARY(inverseSqrt, 1.0f / __builtin_sqrtf(x))
+LW_UNARY(sqrt, sqrt_rn(x))
+LW_UNARY(inverseSqrt, 1.0f / sqrt_rn(x))
 LW_UNARY(abs, __builtin_fabsf(x))
 LW_UNARY(floor, __builtin_floorf(x))
 LW_UNARY(ceil, __builtin_ceilf(x))
@@ -243,7 +268,7 @@ __device__ __forceinline__ vec3 refraction_impl(float t_in, float t_out, vec3 v_
     float eta = quot(eta_in, eta_out);
     float k = 1.0f - eta * eta * (1.0f - cosi * cosi);
     if (k < 0.0f) return reflect(v_in, n_ref);  // total internal reflection
-    return eta * v_in + (eta * cosi - __builtin_sqrtf(k)) * n_ref;
+    return eta * v_in + (eta * cosi - sqrt_rn(k)) * n_ref;
 }
 __device__ __forceinline__ vec3 refraction(float t_in, float t_out, vec3 v_in, vec3 n) {
     return refraction_impl<false>(t_in, t_out, v_in, n);

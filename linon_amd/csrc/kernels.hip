@@ -203,6 +203,15 @@ __global__ void __launch_bounds__(256) div_selftest_random_kernel(unsigned long 
     if (bad) atomicAdd(mism, (unsigned long long)bad);
 }
 
+// every float x whose bit pattern has the top byte `hi`: sqrt_rn(x) against the compiler's
+// correctly rounded square root
+__global__ void __launch_bounds__(256) sqrt_selftest_kernel(unsigned hi, unsigned long long* mism) {
+    const unsigned lo = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lo >= (1u << 24)) return;
+    const float x = __uint_as_float((hi << 24) | lo);
+    if (!same_or_nan(lw::sqrt_rn(x), __builtin_sqrtf(x))) atomicAdd(mism, 1ull);
+}
+
 // refraction_in_range against refraction: temperatures uniform in [-250, 1e4] (one in 64 at an
 // end of the range, one in 1024 NaN), directions and normals with components in [-1.5, 1.5]
 __device__ __forceinline__ float rand_unit(unsigned long long r) { return (float)(r >> 40) * 0x1p-24f; }
@@ -265,6 +274,23 @@ hipError_t run_fma_rate(int cus, double* tflops) {
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(d);
     if (e == hipSuccess) *tflops = (double)blocks * threads * iters * 16.0 * 2.0 / (ms * 1e-3) / 1e12;
+    return e;
+}
+
+hipError_t run_sqrt_selftest(unsigned long long* mism, unsigned long long* cases) {
+    unsigned long long* d = nullptr;
+    hipError_t e = hipMalloc(&d, sizeof(unsigned long long));
+    if (e != hipSuccess) return e;
+    e = hipMemset(d, 0, sizeof(unsigned long long));
+    for (unsigned hi = 0; hi < 256 && e == hipSuccess; ++hi)
+        hipLaunchKernelGGL(sqrt_selftest_kernel, dim3((1u << 24) / 256), dim3(256), 0, nullptr, hi, d);
+    if (e == hipSuccess) e = hipGetLastError();
+    unsigned long long h = 0;
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    *mism = h;
+    *cases = 1ull << 32;
     return e;
 }
 

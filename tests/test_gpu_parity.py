@@ -321,3 +321,13 @@ def test_shared_denominator_division_is_ieee():
     print(res, flush=True)
     assert res["comparisons"] > 20_000_000_000
     assert res["recip_mismatches"] == 0 and res["quotient_mismatches"] == 0, res
+
+
+def test_square_root_is_ieee():
+    """The IEEE build's square root (the hardware root plus a one-ulp correction, inputs below
+    2^-96 rescaled) equals the compiler's correctly rounded square root at every float."""
+    print("[case] sqrt self-test", flush=True)
+    res = L.selftest_sqrt()
+    print(res, flush=True)
+    assert res["comparisons"] == 1 << 32
+    assert res["mismatches"] == 0, res
