@@ -78,6 +78,72 @@ def reference_startup_scene():
     return v, f, get_center(v)
 
 
+def parse_obj(text: str):
+    """The first model of a Wavefront OBJ text as (verts f32 [n, 3], faces u32 [m, 3]), the
+    arrays linon's drop handler passes on (src/main.rs:139-157: tobj::load_obj with
+    triangulate, then models[0].mesh.positions / indices).
+
+    * A new model starts at an `o` or `g` line once the current one has faces.
+    * Face corners may be `p`, `p/t`, `p//n` or `p/t/n`; indices are 1-based, and negative ones
+      count back from the last vertex read so far.
+    * Polygons become triangle fans (0, i, i + 1).
+    * A model's vertices are the positions its faces use, renumbered in order of first use.
+
+    tobj itself (a crate the reference pins but does not vendor) cannot run here.  Its exact
+    vertex order within a model changes no rendered output: normalize_vertices uses only the
+    extent, and faces keep the file's order, which is the hit id.
+    """
+    positions = []
+    models, cur = [], []
+    for raw in text.splitlines():
+        line = raw.split("#", 1)[0].strip()
+        if not line:
+            continue
+        tok = line.split()
+        if tok[0] == "v":
+            positions.append([float(x) for x in tok[1:4]])
+        elif tok[0] == "f":
+            corners = []
+            for c in tok[1:]:
+                i = int(c.split("/")[0])
+                if i == 0:
+                    raise ValueError(f"OBJ: index 0 in {raw!r}")
+                i = i - 1 if i > 0 else len(positions) + i
+                if not 0 <= i < len(positions):
+                    raise ValueError(f"OBJ: vertex index out of range in {raw!r}")
+                corners.append(i)
+            if len(corners) < 3:
+                raise ValueError(f"OBJ: face with fewer than 3 corners: {raw!r}")
+            for k in range(1, len(corners) - 1):
+                cur.append((corners[0], corners[k], corners[k + 1]))
+        elif tok[0] in ("o", "g") and cur:
+            models.append(cur)
+            cur = []
+    if cur:
+        models.append(cur)
+    if not models:
+        raise ValueError("OBJ: no faces")
+    first = models[0]
+    remap = {}
+    for tri in first:
+        for i in tri:
+            remap.setdefault(i, len(remap))
+    verts = np.zeros((len(remap), 3), np.float32)
+    for g, l in remap.items():
+        verts[l] = positions[g]
+    faces = np.array([[remap[i] for i in tri] for tri in first], dtype=np.uint32)
+    return verts, faces
+
+
+def load_obj_scene(path):
+    """A dropped OBJ file as linon loads it (src/application.rs:199-224): the first model,
+    normalised by normalize_vertices.  Returns (verts, faces, center)."""
+    with open(path, "r", encoding="utf-8", errors="replace") as fh:
+        v, f = parse_obj(fh.read())
+    v = normalize_vertices(v)
+    return v, f, get_center(v)
+
+
 def cornell_box_centered():
     """The start-up scene translated so its bounding-box centre is the origin (the camera
     configurations look at the origin)."""

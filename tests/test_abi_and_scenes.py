@@ -95,3 +95,33 @@ def test_splitmix64_reference_values():
     # SplitMix64 from seed 0: published first outputs
     z = scenes.splitmix64(0, 3)
     assert [int(x) for x in z] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_parse_obj_first_model_fans_and_renumbers(tmp_path):
+    """OBJ ingest: quads become fans, v/vt/vn corners and negative indices resolve, only the
+    first model is taken and its vertices are renumbered in order of first use."""
+    text = """# a unit square and a triangle in two objects
+v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+vt 0 0
+vn 0 0 1
+o square
+f 1/1/1 2//1 3 4/1
+o tri
+v 5 5 5
+f -1 1 2
+"""
+    v, f = scenes.parse_obj(text)
+    assert f.tolist() == [[0, 1, 2], [0, 2, 3]]
+    assert v.tolist() == [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]]
+    p = tmp_path / "m.obj"
+    p.write_text("v 2 0 0\nv 0 4 0\nv 0 0 0\nv 9 9 9\nf 3 1 2\n")
+    v2, f2, c2 = scenes.load_obj_scene(p)
+    # vertices in first-use order (3, 1, 2); the unused vertex 4 is dropped before normalising
+    assert f2.tolist() == [[0, 1, 2]]
+    np.testing.assert_array_equal(v2, scenes.normalize_vertices(np.array([[0, 0, 0], [2, 0, 0], [0, 4, 0]], np.float32)))
+    np.testing.assert_array_equal(c2, scenes.get_center(v2))
+    with pytest.raises(ValueError):
+        scenes.parse_obj("v 0 0 0\nf 1 2 3\n")
