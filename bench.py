@@ -96,6 +96,23 @@ def cpu_baseline(wl, verts, faces, budget_s: float) -> dict:
             "frame_ms_extrapolated": dt / (n * len(wl.fields)) * W * H * 1e3}
 
 
+def copy_calibration() -> int:
+    """1 GiB device-to-device copy by a plain copy kernel.  A 2 GiB fill in between evicts the
+    source from the 256 MiB last-level cache, so the copy reads and writes HBM.  Profiled with
+    the FETCH_SIZE and WRITE_SIZE counters, it gives the factor between each counter and bytes."""
+    import torch
+    n = 256 << 20  # floats: 1 GiB
+    a = torch.ones(n, device="cuda")
+    evict = torch.zeros(2 * n, device="cuda")
+    evict.fill_(2.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied {n * 4} bytes")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,7 +126,12 @@ def main() -> int:
     ap.add_argument("--surrogate-steps", type=int, default=1500)
     ap.add_argument("--surrogate-fields", default="swirl,lens,hot_air,warp")
     ap.add_argument("--skip-stress", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device (the reference transfer that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
     a = ap.parse_args()
+    if a.copy_calibration:
+        return copy_calibration()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the ray-cast pass has no multi-GPU path "
               "(rays are independent: it would shard by image tiles or by views, exchanging only "
