@@ -206,7 +206,7 @@ def main() -> int:
             pass
 
     out = {
-        "metric": "RK4 ray-steps/s + 1080p frame ms; surrogate vs plain RK4 at <=1e-3 exit error",
+        "metric": "RK4 ray-steps/s + 1080p frame ms; surrogate vs plain RK4 at \u22641e-3 exit error",
         "value": value,
         "unit": "RK4 ray-steps/s",
         "n_gpus": 1,
@@ -290,7 +290,7 @@ def one_graph_ms(view, fields) -> float:
 
 def bench_stress() -> dict:
     """3840x2160 at h = 0.001 (5001 RK4 iterations per ray) against the Cornell box plus the
-    1,310,720-face level-8 icosphere, lens and hot-air, overlay on, 8 sample rays x 64 records:
+    1,310,720-face level-8 icosphere, lens and hot-air, overlay on, 8 outline rays x 64 records:
     one warm-up and one timed frame per field, then both frames in one captured HIP graph."""
     wl = configs.CONFIG5
     verts, faces = wl.mesh()
@@ -314,7 +314,9 @@ def bench_stress() -> dict:
         out["per_field"][name] = {"raycast_ms": r_ms, "frame_ms": float(pms[0]), "rk4_steps": c["rk4_steps"],
                                   "triangle_tests": c["triangle_tests"],
                                   "gsteps_per_s": c["rk4_steps"] / (r_ms * 1e-3) / 1e9,
-                                  "tflops": fl / (r_ms * 1e-3) / 1e12}
+                                  "tflops": fl / (r_ms * 1e-3) / 1e12,
+                                  # outline searches (of 8) that found a highlighted texel
+                                  "outline_rays_found": int(view.read_outline()[:, 2].sum())}
         log(f"stress {name}: {rms[0]:.1f} ms")
     # both fields' frames captured into one HIP graph on a caller's stream and replayed (the
     # graph keeps each capture's field; the frames share the output buffers)

@@ -36,6 +36,7 @@ class Workload:
     adaptive: bool = False   # fixed-step RK4 ("step h = ..."); see the module docstring
     samples: bool = True
     samples_per_ray: int = 100
+    outline_rays: bool = False  # sample rays start at the outermost high-lambda pixels
 
     def camera(self) -> Camera:
         return Camera.look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), fov_deg=60.0)
@@ -48,7 +49,7 @@ class Workload:
         return Integrator.make(h_initial=self.h, t_max=self.t_max, adaptive=self.adaptive,
                                max_steps=self.max_steps, mode=MODE_REFERENCE,
                                samples_per_ray=self.samples_per_ray if self.samples else 0,
-                               sample_steps=5000 if self.samples else 0)
+                               sample_steps=5000 if self.samples else 0, outline_rays=self.outline_rays)
 
     def mesh(self):
         if self.level is None:
@@ -63,10 +64,11 @@ CONFIG1 = Workload("plain-rk4-256-swirl-cornell", 256, 256, ("swirl",), 0.01, 10
 CONFIG2 = Workload("plain-rk4-overlay-1080p-4fields-cornell+ico20k", 1920, 1080,
                    ("swirl", "lens", "hot_air", "warp"), 0.005, 4000, True, 5)
 # Stress: 3840x2160, h = 0.001 (5001 RK4 iterations per ray under t <= 5), the Cornell box plus
-# the level-8 displaced icosphere (1,310,720 faces), lens and hot-air, overlay, 8 sample rays
-# recording 64 positions each.
+# the level-8 displaced icosphere (1,310,720 faces), lens and hot-air, overlay, 8 outline rays
+# (started at the outermost pixels the previous frame's overlay highlighted) recording 64
+# positions each.
 CONFIG5 = Workload("stress-4k-h0.001-lens+hotair-cornell+ico1.3M", 3840, 2160, ("lens", "hot_air"), 0.001,
-                   20000, True, 8, samples_per_ray=64)
+                   20000, True, 8, samples_per_ray=64, outline_rays=True)
 
 
 def setup_view(view, wl: Workload, field) -> None:
