@@ -125,3 +125,15 @@ f -1 1 2
     np.testing.assert_array_equal(c2, scenes.get_center(v2))
     with pytest.raises(ValueError):
         scenes.parse_obj("v 0 0 0\nf 1 2 3\n")
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with status 2 and says so, before any GPU
+    work."""
+    import subprocess
+    import sys
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
