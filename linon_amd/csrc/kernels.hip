@@ -108,8 +108,8 @@ hipError_t launch_clear_u64(unsigned long long* p, int n, hipStream_t s) {
     return hipLaunchKernel((const void*)&clear_u64_kernel, dim3((n + 63) / 64), dim3(64), args, 0, s);
 }
 
-// ---- self-test of lw::div3_rn / lw::div_rn and the in-range quotient helpers against the IEEE
-// division (IEEE build only) ----
+// ---- IEEE build only: self-tests of its arithmetic (the reciprocal-based quotients against
+// IEEE division, sqrt_rn against the compiler's square root) and the FMA-rate measurement ----
 #ifndef LINON_FAST_MATH
 __device__ __forceinline__ unsigned fbits(float x) { return __float_as_uint(x); }
 
