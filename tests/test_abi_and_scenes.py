@@ -48,6 +48,27 @@ def test_struct_layouts_match_the_reference_uniforms():
     assert L.Camera.view_matrix.offset == 48
 
 
+def test_null_context_is_an_argument_error():
+    """Every entry point that takes a context rejects a NULL one with LINON_ERR_ARGUMENT before
+    it touches the device (so this runs without a GPU); destroy(NULL) is a no-op and
+    last_error(NULL) still returns a message."""
+    lib = L.load_library()
+    text = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "linon.h").read_text(), flags=re.S)
+    decls = re.findall(r"linon_status\s+(linon_[a-z_0-9]+)\s*\(((?:const )?linon_context\*[^;]*)\);", text)
+    assert len(decls) > 30
+    for name, params in decls:
+        f = getattr(lib, name)
+        f.restype, f.argtypes = C.c_int, None
+        assert f(*([None] * (params.count(",") + 1))) == 1, name   # LINON_ERR_ARGUMENT
+    lib.linon_destroy(None)
+    err = lib.linon_last_error
+    err.restype, err.argtypes = C.c_char_p, [C.c_void_p]
+    assert err(None)
+    # the translator entry points take no context and reject a NULL body the same way
+    assert lib.linon_wgsl_translate(None, None, 0, None) == 1
+    assert lib.linon_wgsl_compile_check(None, None, 0) == 1
+
+
 def test_create_without_gpu_fails_cleanly():
     import os
     if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK | os.W_OK):
