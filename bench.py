@@ -62,8 +62,55 @@ def kernel_flops(field: str, c: dict) -> float:
             + c["triangle_tests"] * FLOP_TRIANGLE)
 
 
-def cpu_baseline(wl, verts, faces, budget_s: float) -> dict:
-    """FP32 CPU restatement on a strided sample of each field's pixels."""
+def _rel_err(a, b) -> np.ndarray:
+    """Per-pixel exit-point error max_i |a_i - b_i| / max(1, |b_i|) over x, y, z."""
+    a = np.asarray(a, np.float64)[:, :3]
+    b = np.asarray(b, np.float64)[:, :3]
+    return (np.abs(a - b) / np.maximum(1.0, np.abs(b))).max(axis=1)
+
+
+def _quantiles(e: np.ndarray) -> dict:
+    return {"q50": float(np.quantile(e, 0.5)), "q99": float(np.quantile(e, 0.99)), "max": float(e.max()),
+            "frac_within_1e-4": float((e <= 1e-4).mean())}
+
+
+def frame_accuracy(wl, verts, faces, frames: dict, sample: np.ndarray, f32: dict, n64: int, threads: int) -> dict:
+    """The GPU's full-size frames at the sampled pixels against the FP32 oracle, and both against
+    the FP64 oracle on the first n64 of them.  `frames[field]` = (mapping, hit faces, lambda)
+    read back from the GPU; `f32[field]` = the FP32 oracle's render of `sample`.  Pixels whose
+    lambda is below 0.0955 (where the overlay highlights nothing) are reported on their own."""
+    from oracle import oracle as O
+
+    cam = O.as_oracle_struct(wl.camera(), O.CameraIn)
+    st = O.as_oracle_struct(wl.settings(), O.SettingsIn)
+    ig = O.as_oracle_struct(wl.integrator(), O.IntegratorIn)
+    W, H = wl.width, wl.height
+    rows, cols = H - 1 - sample[:, 1], sample[:, 0]   # invocation (gx, gy) -> texel (gx, H-1-gy)
+    sub = sample[:n64]
+    out = {"pixels": int(len(sample)), "pixels_f64": int(len(sub)),
+           "error": "max over x, y, z of |exit - exit_ref| / max(1, |exit_ref|)"}
+    for name in wl.fields:
+        mapping, hit, lam = frames[name]
+        g_map, g_hit, g_lam = mapping[rows, cols], hit[rows, cols], lam[rows, cols]
+        ref = f32[name]
+        low = (g_lam >= 0) & (g_lam < 0.0955)
+        e32 = _rel_err(g_map, ref["mapping"])
+        r64 = O.render_pixels(cam, st, ig, L.FIELDS[name], verts, faces, W, H, sub, threads, precision="f64")
+        out[name] = {
+            "gpu_vs_f32_oracle": {"hit_agree": float((g_hit == ref["hit"]).mean()),
+                                  "hit_agree_low_lambda": float((g_hit[low] == ref["hit"][low]).mean()) if low.any() else None,
+                                  "low_lambda_pixels": int(low.sum()),
+                                  **_quantiles(e32),
+                                  "low_lambda": _quantiles(e32[low]) if low.any() else None},
+            "vs_f64_oracle": {"gpu": _quantiles(_rel_err(g_map[:n64], r64["mapping"])),
+                              "f32_oracle": _quantiles(_rel_err(ref["mapping"][:n64], r64["mapping"]))},
+        }
+    return out
+
+
+def cpu_baseline(wl, verts, faces, budget_s: float, frames: dict | None = None) -> dict:
+    """FP32 CPU restatement on a strided sample of each field's pixels.  With the GPU's frames
+    given, the same sample also yields the frame_accuracy() report (outside the timed region)."""
     from oracle import oracle as O
 
     threads = max(1, min(16, O.threads_available()))
@@ -84,16 +131,20 @@ def cpu_baseline(wl, verts, faces, budget_s: float) -> dict:
     n = int(max(threads, min(len(allpix), budget_s / max(per_ray, 1e-9) / len(wl.fields))))
     sample = allpix[order[:n]]
     steps = 0
+    f32 = {}
     t0 = time.perf_counter()
     for name in wl.fields:
-        r = O.render_pixels(cam, st, ig, L.FIELDS[name], verts, faces, W, H, sample, threads)
-        steps += r["rk4_steps"]
+        f32[name] = O.render_pixels(cam, st, ig, L.FIELDS[name], verts, faces, W, H, sample, threads)
+        steps += f32[name]["rk4_steps"]
     dt = time.perf_counter() - t0
-    return {"value": steps / dt, "unit": "RK4 ray-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} random pixels (seed 0) of each of the 4 fields at 1920x1080, FP32 CPU restatement "
-                      f"of main_view.wgsl (brute force over {len(faces)} faces per pre-hit step), "
-                      f"{steps} RK4 steps in {dt:.1f} s on {threads} threads",
-            "frame_ms_extrapolated": dt / (n * len(wl.fields)) * W * H * 1e3}
+    out = {"value": steps / dt, "unit": "RK4 ray-steps/s", "cores": threads, "kind": "port",
+           "sample": f"{n} random pixels (seed 0) of each of the 4 fields at 1920x1080, FP32 CPU restatement "
+                     f"of main_view.wgsl (brute force over {len(faces)} faces per pre-hit step), "
+                     f"{steps} RK4 steps in {dt:.1f} s on {threads} threads",
+           "frame_ms_extrapolated": dt / (n * len(wl.fields)) * W * H * 1e3}
+    if frames:
+        out["accuracy"] = frame_accuracy(wl, verts, faces, frames, sample, f32, min(n, 512), threads)
+    return out
 
 
 def copy_calibration() -> int:
@@ -148,11 +199,15 @@ def main() -> int:
 
     # warm-up also records each field's (deterministic) work counters
     counters = {}
+    frames = {}   # the last warm-up frame of each field, for the accuracy report
     for w in range(max(1, a.warmup)):
+        last = w == max(1, a.warmup) - 1
         for name in wl.fields:
             view.set_field(name)
             view.render()
             counters[name] = view.counters()
+            if last and not a.no_cpu_baseline:
+                frames[name] = (view.read_mapping(), view.read_hit_faces(), view.read_lyapunov())
         log(f"warmup {w + 1}/{max(1, a.warmup)} done")
 
     view.synchronize()
@@ -245,7 +300,9 @@ def main() -> int:
     out["other_workloads"] = extra
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(wl, verts, faces, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(wl, verts, faces, a.cpu_seconds, frames)
+        # exit-point error of the GPU frames and of the FP32 oracle, both against FP64
+        out["accuracy"] = out["cpu_baseline"].pop("accuracy", None)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out, default=_json_scalar), flush=True)
