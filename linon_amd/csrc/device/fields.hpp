@@ -8,39 +8,38 @@
 
 namespace lw {
 
+// A functor may also define first_stage(p, v0, v, t) = (*this)(p, p, v0, v, t), bit for bit,
+// with less work; the integrators call it for the first RK4 stage, where p_prev == p.
 template <bool Spherical, bool Sigmoid>
 struct FieldMirage {  // src/functions.rs:36-96
-    __device__ __forceinline__ vec3 operator()(vec3 p_prev, vec3 p, vec3, vec3 v, float) const {
+    // distance from the hot source: the sphere's centre or the plane
+    __device__ __forceinline__ static float distance_to_source(vec3 q) {
+        if (Spherical) return length(q - vec3f(-0.5f, 0.5f, -0.5f));
+        return point_plane_distance(q, vec3f(0.0f, 1.0f, 0.0f), vec3f(0.0f, 0.1f, 0.0f));
+    }
+    __device__ __forceinline__ static vec3 normal_at(vec3 p) {
+        if (Spherical) return normalize(p - vec3f(-0.5f, 0.5f, -0.5f));
+        return vec3f(0.0f, 1.0f, 0.0f);
+    }
+    __device__ __forceinline__ static float temperature(float dist) {  // in [15, 200]
         const float t_env = 15.0f;
         const float t_src = Spherical ? 200.0f : 30.0f;
         const float max_dist = Spherical ? 0.25f : 0.01f;
-        float dist_in, dist_out;
-        vec3 normal;
-        if (Spherical) {
-            const vec3 center = vec3f(-0.5f, 0.5f, -0.5f);
-            vec3 center_dest = p - center;
-            normal = normalize(center_dest);
-            dist_in = length(p_prev - center);
-            dist_out = length(center_dest);
-        } else {
-            const vec3 plane_p0 = vec3f(0.0f, 0.1f, 0.0f);
-            const vec3 plane_n = vec3f(0.0f, 1.0f, 0.0f);
-            normal = plane_n;
-            dist_in = point_plane_distance(p_prev, plane_n, plane_p0);
-            dist_out = point_plane_distance(p, plane_n, plane_p0);
-        }
-        float part_in, part_out;
+        float part;
         if (Sigmoid) {
             // arguments >= -6 (distances are >= 0): sigmoid_floor equals sigmoid here
-            part_in = clamp(0.0f, 1.0f, sigmoid_floor(dist_in / max_dist * 12.0f - 6.0f));
-            part_out = clamp(0.0f, 1.0f, sigmoid_floor(dist_out / max_dist * 12.0f - 6.0f));
+            part = clamp(0.0f, 1.0f, sigmoid_floor(dist / max_dist * 12.0f - 6.0f));
         } else {
-            part_in = clamp(0.0f, 1.0f, dist_in / max_dist);
-            part_out = clamp(0.0f, 1.0f, dist_out / max_dist);
+            part = clamp(0.0f, 1.0f, dist / max_dist);
         }
-        float t_in = part_in * t_env + (1.0f - part_in) * t_src;
-        float t_out = part_out * t_env + (1.0f - part_out) * t_src;
-        return refraction_in_range(t_in, t_out, v, normal);  // t_in, t_out in [15, 200]
+        return part * t_env + (1.0f - part) * t_src;
+    }
+    __device__ __forceinline__ vec3 operator()(vec3 p_prev, vec3 p, vec3, vec3 v, float) const {
+        return refraction_in_range(temperature(distance_to_source(p_prev)), temperature(distance_to_source(p)), v,
+                                   normal_at(p));
+    }
+    __device__ __forceinline__ vec3 first_stage(vec3 p, vec3, vec3 v, float) const {
+        return refraction_in_range_same(temperature(distance_to_source(p)), v, normal_at(p));
     }
 };
 using FieldMirageSpherical = FieldMirage<true, false>;
@@ -97,6 +96,10 @@ struct FieldHotAir {  // refraction(T(p_prev), T(p), v, normalize(p - c)), T = 2
     __device__ __forceinline__ vec3 operator()(vec3 p_prev, vec3 p, vec3, vec3 v, float) const {
         const vec3 c = vec3f(0.0f, -0.5f, 0.0f);
         return refraction_in_range(temperature(p_prev), temperature(p), v, normalize(p - c));
+    }
+    __device__ __forceinline__ vec3 first_stage(vec3 p, vec3, vec3 v, float) const {
+        const vec3 c = vec3f(0.0f, -0.5f, 0.0f);
+        return refraction_in_range_same(temperature(p), v, normalize(p - c));
     }
 };
 struct FieldWarp {  // v0 + 0.3 * (sin 4p.y, 0, sin 4p.x)

@@ -316,6 +316,19 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
     return x * q + min(x, r) + i;
 }
 
+// The first RK4 stage, f(p, p, v0, v, t): through the field's first_stage() where it defines
+// one (the same result with less work), else through the field itself.  The int argument
+// prefers the first overload wherever it is well-formed.
+template <class F>
+__device__ __forceinline__ auto rk4_stage1(const F& f, vec3 p, vec3 v0, vec3 v, float t, int)
+    -> decltype(f.first_stage(p, v0, v, t)) {
+    return f.first_stage(p, v0, v, t);
+}
+template <class F>
+__device__ __forceinline__ vec3 rk4_stage1(const F& f, vec3 p, vec3 v0, vec3 v, float t, long) {
+    return f(p, p, v0, v, t);
+}
+
 // The 8 sample rays (src/main_view.wgsl:281-335,470-479): fixed h = 0.001, no scene queries,
 // the position before every stride-th step recorded as {(p, 1), (colour, 0.5)}.
 template <class F>
@@ -355,7 +368,7 @@ __device__ void sample_rays(const RaycastParams& P, const CameraFrame& cam, int 
     const int stride = P.sample_steps / nsamp;
     float4* out = reinterpret_cast<float4*>(P.ray_samples);
     for (int i = 0; i < P.sample_steps; ++i) {
-        vec3 k1 = field(p, p, start_dir, d, t);
+        vec3 k1 = rk4_stage1(field, p, start_dir, d, t, 0);
         vec3 k2 = field(p, rn::add(p, rn::mul(hh, k1)), start_dir, k1, rn::add(t, hh));
         vec3 k3 = field(p, rn::add(p, rn::mul(hh, k2)), start_dir, k2, rn::add(t, hh));
         vec3 k4 = field(p, rn::add(p, rn::mul(h, k3)), start_dir, k3, rn::add(t, h));
@@ -465,7 +478,7 @@ __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, co
         // the field sees opaque copies of the ray state the integrator itself computes with
         // (position and direction; see opaque())
         const vec3 pf = opaque(p);
-        vec3 k1 = field(pf, pf, start_dir, opaque(r.d), r.t);
+        vec3 k1 = rk4_stage1(field, pf, start_dir, opaque(r.d), r.t, 0);
         vec3 k2 = field(pf, rn::add(p, rn::mul(hh, k1)), start_dir, k1, rn::add(r.t, hh));
         vec3 k3 = field(pf, rn::add(p, rn::mul(hh, k2)), start_dir, k2, rn::add(r.t, hh));
         vec3 k4 = field(pf, rn::add(p, rn::mul(h, k3)), start_dir, k3, rn::add(r.t, h));

@@ -114,15 +114,19 @@ __device__ __forceinline__ float length(vec3 a) { return sqrt_rn(dot(a, a)); }
 __device__ __forceinline__ float length(float a) { return __builtin_fabsf(a); }
 // v / s for callers whose components are bounded through the denominator: every |v_i| is at
 // most 2^61 whenever 2^-60 <= s <= 2^60 (normalize: |v_i| <= length(v) to an ulp; the lens
-// field: |0.3 p_i| <= 0.3 r <= 0.3 s^(1/3) with s = max(r^3, 0.001)).  The IEEE build's range
-// test then needs only s and the smallest component magnitude; NaN or infinite components make
-// s NaN or infinite there, which fails the test as in div3_rn.  Same result as v / s.
+// field: |0.3 p_i| <= 0.3 r <= 0.3 s^(1/3) with s = max(r^3, 0.001)), and s >= 2^-61 wherever
+// s is a number and the smallest |v_i| is at least 2^-60 (normalize: length(v) >= |v_i| to an
+// ulp; lens: s >= 0.001).  With s in [2^-61, 2^60] and every |v_i| in [2^-60, 2^61] no
+// intermediate of the quotients comes near the subnormal or overflow range, so the IEEE
+// build's range test needs only s <= 2^60 and the smallest component magnitude; NaN or
+// infinite components make s NaN or infinite there, which fails the test as in div3_rn.  Same
+// result as v / s.
 __device__ __forceinline__ vec3 div3_rn_bounded(vec3 v, float s) {
 #ifdef LINON_FAST_MATH
     return v / s;
 #else
     const float lo = __builtin_fminf(__builtin_fminf(__builtin_fabsf(v.x), __builtin_fabsf(v.y)), __builtin_fabsf(v.z));
-    if (s >= 0x1p-60f && s <= 0x1p60f && lo >= 0x1p-60f) {
+    if (s <= 0x1p60f && lo >= 0x1p-60f) {
         const float y = recip_rn(s);
         return {quot_rn(v.x, s, y), quot_rn(v.y, s, y), quot_rn(v.z, s, y)};
     }
@@ -277,6 +281,37 @@ __device__ __forceinline__ vec3 refraction(float t_in, float t_out, vec3 v_in, v
 // [20, 100], the mirages' in [15, 200]); bit for bit the same result.
 __device__ __forceinline__ vec3 refraction_in_range(float t_in, float t_out, vec3 v_in, vec3 n) {
     return refraction_impl<true>(t_in, t_out, v_in, n);
+}
+// refraction_in_range(t, t, v_in, n): one temperature on both sides, as in the first RK4 stage
+// of a field that compares T(p_prev) with T(p).  The index ratio n(t) / n(t) is then exactly 1
+// (a correctly rounded x / x), so its quotients and every product with it drop out:
+//   k = fma(-(eta eta), fma(-cosi, cosi, 1), 1)  ->  1 - fma(-cosi, cosi, 1)
+//   X = fma(eta, cosi, -sqrt(k))                 ->  cosi - sqrt(k)
+//   fma(v_in, eta, RN(X n))                      ->  v_in + RN(X n)
+// which is how refraction_impl's expression compiles, so the result is the same bit for bit.  A
+// NaN t gives NaN in every component, as its NaN ratio does there (cosi - (t - t) is cosi for
+// a finite t).  The WGSL-accuracy build divides approximately, where n(t) / n(t) need not be
+// 1, and keeps the general form.
+__device__ __forceinline__ vec3 refraction_in_range_same(float t, vec3 v_in, vec3 n) {
+#ifdef LINON_FAST_MATH
+    return refraction_impl<true>(t, t, v_in, n);
+#else
+    float cosi = clamp(-1.0f, 1.0f, dot(v_in, n)) - (t - t);
+    vec3 n_ref = n;
+    if (cosi < 0.0f) {
+        cosi = -cosi;
+    } else {
+        n_ref = -n;
+    }
+    const float k = 1.0f - __builtin_fmaf(-cosi, cosi, 1.0f);
+    if (k < 0.0f) return reflect(v_in, n_ref);
+    const float x = cosi - sqrt_rn(k);
+    auto axpy = [](float a, float b, float c) {
+#pragma clang fp contract(off)
+        return a * b + c;
+    };
+    return {axpy(x, n_ref.x, v_in.x), axpy(x, n_ref.y, v_in.y), axpy(x, n_ref.z, v_in.z)};
+#endif
 }
 __device__ __forceinline__ float point_plane_distance(vec3 p, vec3 n, vec3 p0) {
     float d = dot(p0, n);
