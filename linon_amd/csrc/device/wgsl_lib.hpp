@@ -87,10 +87,13 @@ __device__ __forceinline__ vec3 operator-(float s, vec3 a) { return {s - a.x, s 
 // residuals could underflow, are scaled by 2^32 first and the root by 2^-16 after.  Zeros,
 // +inf and NaN pass through the correction unchanged, so unlike the compiler's expansion this
 // one has no separate special-case select.  linon_selftest_sqrt compares it with the
-// compiler's square root at every float.
+// compiler's square root at every float.  The WGSL-accuracy build takes v_sqrt_f32 alone
+// (within 1 ulp for normal inputs): the compiler's approximate sqrt adds a rescaling for
+// inputs below 2^-96, five instructions per root, and WGSL bounds sqrt only as
+// inverseSqrt does, for normal values.
 __device__ __forceinline__ float sqrt_rn(float x) {
 #ifdef LINON_FAST_MATH
-    return __builtin_sqrtf(x);
+    return __builtin_amdgcn_sqrtf(x);
 #else
 #pragma clang fp contract(off)
     const bool tiny = x < 0x1p-96f;
