@@ -189,7 +189,7 @@ def main() -> int:
               "the overlay's border rows between tiles)", file=sys.stderr)
         return 2
 
-    wl = configs.CONFIG2
+    wl = configs.OVERLAY_1080P
     verts, faces = wl.mesh()
     view = L.MainView(wl.width, wl.height)
     view.update_model(verts, faces)
@@ -295,8 +295,8 @@ def main() -> int:
         extra["stress_4k"] = bench_stress()
         if not a.skip_surrogate:
             log("4K stress surrogate ...")
-            extra["stress_4k"]["surrogate"] = bench_surrogate(a.surrogate_steps, configs.CONFIG5.fields,
-                                                              configs.CONFIG5)
+            extra["stress_4k"]["surrogate"] = bench_surrogate(a.surrogate_steps, configs.STRESS_4K.fields,
+                                                              configs.STRESS_4K)
     out["other_workloads"] = extra
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
@@ -349,7 +349,7 @@ def bench_stress() -> dict:
     """3840x2160 at h = 0.001 (5001 RK4 iterations per ray) against the Cornell box plus the
     1,310,720-face level-8 icosphere, lens and hot-air, overlay on, 8 outline rays x 64 records:
     one warm-up and one timed frame per field, then both frames in one captured HIP graph."""
-    wl = configs.CONFIG5
+    wl = configs.STRESS_4K
     verts, faces = wl.mesh()
     view = L.MainView(wl.width, wl.height)
     t0 = time.perf_counter()
@@ -429,7 +429,7 @@ def bench_batch(verts, faces, replays: int) -> dict:
             "tuning": tuning.as_dict(), "autotune_full_size_ms": tuned_ms, "autotune_seconds": tune_s}
 
 
-def bench_surrogate(steps: int, fields, wl=configs.CONFIG2) -> dict:
+def bench_surrogate(steps: int, fields, wl=configs.OVERLAY_1080P) -> dict:
     """Per field: train the flow-map surrogate (2^24 uniform entry states labelled by the FP32
     RK4 kernel, batch 65536, Adam lr 1e-3), then render the workload's view (1080p by default)
     without the mesh (mapping + overlay) twice: plain RK4 (escape mode) and surrogate +

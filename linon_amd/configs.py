@@ -1,11 +1,11 @@
 """The benchmark workloads as host-side set-ups of a MainView.
 
-* ``config1``: 256x256, 1 ray/pixel, 60-degree pinhole at distance 3 looking at the origin,
+* ``PLAIN_256_SWIRL``: 256x256, 1 ray/pixel, 60-degree pinhole at distance 3 looking at the origin,
   the Cornell box (centred), swirl field, h = 0.01, at most 1000 RK4 iterations, no overlay.
-* ``config2``: 1920x1080, the Cornell box plus a seeded 20,480-face displaced icosphere,
+* ``OVERLAY_1080P``: 1920x1080, the Cornell box plus a seeded 20,480-face displaced icosphere,
   fields swirl / lens / hot-air / warp each rendered once, h = 0.005, at most 4000 RK4
   iterations, Lyapunov overlay over 3x3 neighbours (delta 1, scaling 50).
-* ``config5``: the stress case, 3840x2160 at h = 0.001 against 1.3 M faces (lens, hot-air).
+* ``STRESS_4K``: the stress case, 3840x2160 at h = 0.001 against 1.3 M faces (lens, hot-air).
 
 Both keep the reference's integration loop `for (; t <= 5.0;)` with `t = t + h` in f32
 (src/main_view.wgsl:244,265); the step cap never binds at these h.  They integrate with the
@@ -60,14 +60,14 @@ class Workload:
         return [FIELDS[f] for f in self.fields]
 
 
-CONFIG1 = Workload("plain-rk4-256-swirl-cornell", 256, 256, ("swirl",), 0.01, 1000, False, None)
-CONFIG2 = Workload("plain-rk4-overlay-1080p-4fields-cornell+ico20k", 1920, 1080,
+PLAIN_256_SWIRL = Workload("plain-rk4-256-swirl-cornell", 256, 256, ("swirl",), 0.01, 1000, False, None)
+OVERLAY_1080P = Workload("plain-rk4-overlay-1080p-4fields-cornell+ico20k", 1920, 1080,
                    ("swirl", "lens", "hot_air", "warp"), 0.005, 4000, True, 5)
 # Stress: 3840x2160, h = 0.001 (5001 RK4 iterations per ray under t <= 5), the Cornell box plus
 # the level-8 displaced icosphere (1,310,720 faces), lens and hot-air, overlay, 8 outline rays
 # (started at the outermost pixels the previous frame's overlay highlighted) recording 64
 # positions each.
-CONFIG5 = Workload("stress-4k-h0.001-lens+hotair-cornell+ico1.3M", 3840, 2160, ("lens", "hot_air"), 0.001,
+STRESS_4K = Workload("stress-4k-h0.001-lens+hotair-cornell+ico1.3M", 3840, 2160, ("lens", "hot_air"), 0.001,
                    20000, True, 8, samples_per_ray=64, outline_rays=True)
 
 
@@ -94,7 +94,7 @@ FIELD_WGSL = {
 
 
 # ---------------------------------------------------------------------------------------------
-# 64 views in one persistent launch per field (escape mode, uneven path lengths)
+# Batched views: 64 views in one persistent launch per field (escape mode, uneven path lengths)
 # ---------------------------------------------------------------------------------------------
 BATCH_RESOLUTIONS = ((256, 256), (512, 512), (1024, 1024), (1920, 1080), (2048, 2048))
 BATCH_STEPS = (0.01, 0.005, 0.0025)

@@ -38,14 +38,14 @@ def cases():
     yield ("startup_mirage", L.Camera.arcball_default(c), L.Settings.make(overlay_mode=1),
            L.Integrator.make(h_initial=0.02, adaptive=False, max_steps=1000, samples_per_ray=0, sample_steps=0),
            L.FIELDS["mirage_spherical_sigmoid"], v, f, 48, 36)
-    wl = configs.CONFIG1
+    wl = configs.PLAIN_256_SWIRL
     v, f = wl.mesh()
     for name in ("swirl", "lens", "warp", "hot_air"):
         yield (f"cornell_{name}", wl.camera(), wl.settings(),
                L.Integrator.make(h_initial=0.01, adaptive=False, max_steps=1000, samples_per_ray=0, sample_steps=0),
                L.FIELDS[name], v, f, 32, 32)
     v, f = scenes.benchmark_scene(level=1)
-    w2 = configs.CONFIG2
+    w2 = configs.OVERLAY_1080P
     yield ("bench1_rotation", w2.camera(), w2.settings(),
            L.Integrator.make(h_initial=0.02, adaptive=False, max_steps=1000, samples_per_ray=0, sample_steps=0),
            L.FIELDS["rotation"], v, f, 24, 16)

@@ -42,7 +42,7 @@ def test_ragged_sizes_identity_field(view, W, H):
     """The identity field is a straight line: no field arithmetic, so hit faces must agree
     exactly and end points to rounding, on images whose edge tiles are partial, up to the
     largest width and height the context accepts (16384)."""
-    wl = configs.CONFIG1
+    wl = configs.PLAIN_256_SWIRL
     v, f = wl.mesh()
     ig = L.Integrator.make(h_initial=0.01, adaptive=False, max_steps=1000, samples_per_ray=0, sample_steps=0)
     print(f"[case] ragged {W}x{H}", flush=True)
@@ -86,7 +86,7 @@ def test_empty_scene(view):
     ends; swirl bends the rays."""
     v, f = np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32)
     ig = L.Integrator.make(h_initial=0.01, adaptive=False, max_steps=1000, samples_per_ray=0, sample_steps=0)
-    wl = configs.CONFIG1
+    wl = configs.PLAIN_256_SWIRL
     print("[case] empty scene", flush=True)
     g, r = _render_both(view, 24, 16, wl.camera(), wl.settings(), ig, "swirl", v, f)
     assert (g.hit_faces == -1).all() and (r["hit"] == -1).all()

@@ -42,7 +42,7 @@ def test_device_grid_equals_host_grid(mesh, res):
 
 
 def test_device_grid_renders_the_same_frame():
-    wl = configs.CONFIG2
+    wl = configs.OVERLAY_1080P
     v, f = scenes.benchmark_scene(level=4)
     view = L.MainView(96, 64)
     try:

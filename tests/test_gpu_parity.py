@@ -51,7 +51,7 @@ def test_reference_default_scene(view):
 
 @pytest.mark.parametrize("field", ["identity", "swirl", "lens", "hot_air", "warp"])
 def test_config_camera_cornell(view, field):
-    wl = configs.CONFIG1
+    wl = configs.PLAIN_256_SWIRL
     v, f = wl.mesh()
     _case(view, 64, 64, wl.camera(), wl.settings(), wl.integrator(), field, v, f, f"cornell-{field}")
 
@@ -62,7 +62,7 @@ def test_math_builds(view, field, math):
     """Both kernel builds against the oracle on the 64x64 Cornell view: IEEE (correctly rounded
     division and sqrt, full-precision transcendentals) and WGSL accuracy (hardware division,
     sqrt, sin/cos/exp within WGSL's bounds; AUTO's choice for swirl and warp)."""
-    wl = configs.CONFIG1
+    wl = configs.PLAIN_256_SWIRL
     v, f = wl.mesh()
     view.set_math({"ieee": L.MATH_IEEE, "wgsl": L.MATH_WGSL}[math])
     try:
@@ -73,7 +73,7 @@ def test_math_builds(view, field, math):
 
 @pytest.mark.parametrize("field", ["swirl", "lens", "hot_air", "warp"])
 def test_benchmark_scene_small(view, field):
-    wl = configs.CONFIG2
+    wl = configs.OVERLAY_1080P
     v, f = scenes.benchmark_scene(level=2)
     _case(view, 48, 32, wl.camera(), wl.settings(), wl.integrator(), field, v, f, f"bench-scene-{field}")
 
@@ -89,7 +89,7 @@ def test_predefined_fields(view, fid):
 def test_grid_equals_brute_force(view):
     """The grid path (distance-field skip + cell walk) returns the same face, colour and end
     point as the reference's loop over all faces, bit for bit."""
-    wl = configs.CONFIG2
+    wl = configs.OVERLAY_1080P
     v, f = scenes.benchmark_scene(level=3)
     view.resize_texture(96, 64)
     outs = []
@@ -226,7 +226,7 @@ def test_high_accuracy_render(view):
 
 def test_counters_and_errors(view):
     v, f = scenes.cornell_box_centered()
-    wl = configs.CONFIG1
+    wl = configs.PLAIN_256_SWIRL
     view.resize_texture(32, 32)
     run_gpu(view, wl.camera(), wl.settings(), wl.integrator(), "identity", v, f)
     c = view.counters()
@@ -242,11 +242,11 @@ def test_counters_and_errors(view):
 
 
 @pytest.mark.parametrize("field", ["swirl", "lens", "hot_air", "warp"])
-def test_config2_full_resolution_sample(field):
+def test_1080p_full_resolution_sample(field):
     """The 1920x1080 benchmark frame (Cornell box + 20,480-face icosphere, h 0.005): the whole
     frame on the GPU, 1024 random pixels through the FP32 and FP64 oracles; lambda from the
     GPU's own overlay at full resolution."""
-    wl = configs.CONFIG2
+    wl = configs.OVERLAY_1080P
     v, f = wl.mesh()
     with L.MainView(wl.width, wl.height) as big:
         big.update_model(v, f)
@@ -268,7 +268,7 @@ def test_config2_full_resolution_sample(field):
     ref = {"mapping": r32["mapping"][None], "hit": r32["hit"][None], "noise_flag": np.zeros((1, len(px)), np.uint8),
            "color8": g.ray_casting}
     lam = fr.lyapunov[ty, gx][None]
-    info = compare(g, ref, ref64, f"config2-{field}", lam_gpu=lam)
+    info = compare(g, ref, ref64, f"1080p-{field}", lam_gpu=lam)
     assert info["gated"] > 100, info
     assert info["hit_ok"], info
     assert info["exit_pass"], info
@@ -279,7 +279,7 @@ def test_config2_full_resolution_sample(field):
 def test_wgsl_body_matches_predefined_functor(view, field):
     """MainView::reload_shader path: the WGSL body of a benchmark field, translated and compiled
     with hiprtc, renders the same frame as the statically compiled functor."""
-    wl = configs.CONFIG2
+    wl = configs.OVERLAY_1080P
     v, f = scenes.benchmark_scene(level=2)
     view.resize_texture(64, 40)
     try:
