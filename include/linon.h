@@ -312,8 +312,11 @@ linon_status linon_get_surrogate_stats(linon_context* ctx, linon_surrogate_stats
  * against 16 structured denominators and on random_pairs random operand pairs (zeros and
  * out-of-range magnitudes included); lw::sigmoid_floor at every float, and hot air's
  * temperature from it; lw::refraction_in_range against lw::refraction on random_pairs random
- * temperatures in its range, directions and normals.  quotient_mismatches counts all but the
- * reciprocal's.  Writes the mismatch counts and the number of comparisons. */
+ * temperatures in its range, directions and normals, and on the same draws the one-temperature
+ * form lw::refraction_in_range_same and the refracting fields' first RK4 stage (hot air and the
+ * four mirages: first_stage(p, ...) against the field at p_prev = p).  quotient_mismatches
+ * counts all but the reciprocal's.  Writes the mismatch counts and the number of
+ * comparisons. */
 linon_status linon_selftest_division(int device, uint64_t random_pairs, uint64_t* recip_mismatches,
                                      uint64_t* quotient_mismatches, uint64_t* comparisons);
 /* Diagnostics: the IEEE build's square root (lw::sqrt_rn: the hardware root plus a one-ulp

@@ -588,7 +588,8 @@ def measure_fma_rate(device: int = 0) -> float:
 
 def selftest_division(random_pairs: int = 1 << 28, device: int = 0) -> dict:
     """Bit-for-bit check of the IEEE build's reciprocal-based divisions against IEEE division
-    (on the GPU): the shared-denominator division, sigmoid_floor and refraction_in_range.
+    (on the GPU): the shared-denominator division, sigmoid_floor and refraction_in_range, and
+    the refracting fields' first-stage form (first_stage(p, ...) against the field at p_prev = p).
     Returns the mismatch counts of the reciprocal and of the quotients, and the number of
     comparisons."""
     lib = load_library()

@@ -221,6 +221,11 @@ __device__ __forceinline__ float rand_temp(unsigned long long r) {
     if (sel < 16) return (sel & 1) ? -250.0f : 1e4f;
     return -250.0f + 10250.0f * rand_unit(r);
 }
+template <class F>
+__device__ __forceinline__ unsigned first_stage_mismatches(const F& f, lw::vec3 p, lw::vec3 v) {
+    const lw::vec3 a = f.first_stage(p, v, v, 0.0f), b = f(p, p, v, v, 0.0f);
+    return !same_or_nan(a.x, b.x) + !same_or_nan(a.y, b.y) + !same_or_nan(a.z, b.z);
+}
 __global__ void __launch_bounds__(256) div_selftest_refraction_kernel(unsigned long long seed, unsigned long long n,
                                                                       unsigned long long* mism) {
     const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -231,7 +236,17 @@ __global__ void __launch_bounds__(256) div_selftest_refraction_kernel(unsigned l
     const lw::vec3 v = lw::vec3f(3.0f * rand_unit(r[2]) - 1.5f, 3.0f * rand_unit(r[3]) - 1.5f, 3.0f * rand_unit(r[4]) - 1.5f);
     const lw::vec3 nrm = lw::vec3f(3.0f * rand_unit(r[5]) - 1.5f, 3.0f * rand_unit(r[6]) - 1.5f, 3.0f * rand_unit(r[7]) - 1.5f);
     const lw::vec3 a = lw::refraction(t_in, t_out, v, nrm), b = lw::refraction_in_range(t_in, t_out, v, nrm);
-    const unsigned bad = !same_or_nan(a.x, b.x) + !same_or_nan(a.y, b.y) + !same_or_nan(a.z, b.z);
+    unsigned bad = !same_or_nan(a.x, b.x) + !same_or_nan(a.y, b.y) + !same_or_nan(a.z, b.z);
+    // one temperature on both sides: refraction_in_range_same against the general form
+    const lw::vec3 c = lw::refraction_in_range_same(t_in, v, nrm), d = lw::refraction_in_range(t_in, t_in, v, nrm);
+    bad += !same_or_nan(c.x, d.x) + !same_or_nan(c.y, d.y) + !same_or_nan(c.z, d.z);
+    // the refracting fields' first RK4 stage, f.first_stage(p, v0, v, t) = f(p, p, v0, v, t),
+    // with p taken from the random normals (components in [-1.5, 1.5])
+    bad += first_stage_mismatches(lw::FieldHotAir{}, nrm, v);
+    bad += first_stage_mismatches(lw::FieldMirageSpherical{}, nrm, v);
+    bad += first_stage_mismatches(lw::FieldMiragePlane{}, nrm, v);
+    bad += first_stage_mismatches(lw::FieldMirageSphericalSigmoid{}, nrm, v);
+    bad += first_stage_mismatches(lw::FieldMiragePlaneSigmoid{}, nrm, v);
     if (bad) atomicAdd(mism, (unsigned long long)bad);
 }
 #endif
@@ -327,7 +342,7 @@ hipError_t run_division_selftest(unsigned long long random_pairs, unsigned long 
             const unsigned long long m = random_pairs - off < chunk ? random_pairs - off : chunk;
             hipLaunchKernelGGL(div_selftest_refraction_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
                                nullptr, 0x5851F42D4C957F2Dull + off, m, d + 1);
-            n += 3 * m;
+            n += 21 * m;
         }
         for (unsigned long long off = 0; off < random_pairs && e == hipSuccess; off += chunk) {
             const unsigned long long m = random_pairs - off < chunk ? random_pairs - off : chunk;

@@ -315,7 +315,9 @@ def test_shared_denominator_division_is_ieee():
     must equal IEEE division bit for bit: every mantissa of the reciprocal at exponents
     -126..125, every numerator mantissa against 16 structured denominators, 2^28 random pairs
     (zeros, out-of-range and signed operands included), sigmoid_floor at every float and
-    refraction_in_range on 2^28 random temperature pairs and directions."""
+    refraction_in_range on 2^28 random temperature pairs and directions.  On the same draws, the
+    first RK4 stage of hot air and the four mirages, which skips the index ratio n(T)/n(T) = 1,
+    must equal the field evaluated at p_prev = p."""
     print("[case] division self-test", flush=True)
     res = L.selftest_division(random_pairs=1 << 28)
     print(res, flush=True)
