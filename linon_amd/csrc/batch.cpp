@@ -167,8 +167,11 @@ linon_status linon_set_batch(linon_context* ctx, const linon_view* views, int nu
     if (hipSetDevice(c->device) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
         return c->fail(LINON_ERR_HIP, "set_batch: device not ready");
     linon_status st = c->batch_prepare(c->batch, views, num_views);
-    if (st != LINON_OK) return st;
-    return c->batch_capture();
+    if (st == LINON_OK) st = c->batch_capture();
+    // argument and state errors leave the previous batch as it was; a HIP error comes after it
+    // was released, so nothing half-prepared is left to render or read
+    if (st == LINON_ERR_HIP) c->batch.release();
+    return st;
 }
 
 linon_status linon_render_batch(linon_context* ctx, void* stream) {

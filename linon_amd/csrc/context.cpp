@@ -52,6 +52,15 @@ void Context::free_mesh() {
 }
 
 linon_status Context::alloc_images(uint32_t w, uint32_t h) {
+    const linon_status st = alloc_images_at(w, h);
+    if (st != LINON_OK) {  // no half-sized set of buffers: render and the reads then refuse
+        free_images();
+        width = height = 0;
+    }
+    return st;
+}
+
+linon_status Context::alloc_images_at(uint32_t w, uint32_t h) {
     free_images();
     width = w;
     height = h;
@@ -163,6 +172,7 @@ hipEvent_t* Context::next_timing_slot() {
 linon_status Context::render_with(const linon_integrator& ig, hipStream_t s) {
     if (!have_mesh) return fail(LINON_ERR_STATE, "render: no mesh set (linon_set_mesh)");
     if (field_id < 0 && !jit_fn) return fail(LINON_ERR_STATE, "render: no field selected");
+    if (!d_color) return fail(LINON_ERR_STATE, "render: no image buffers (the last linon_resize failed)");
     LN_CHECK_HIP(this, hipSetDevice(device));
     if (!s) s = stream;
     RaycastParams P;
@@ -224,6 +234,7 @@ bool Context::wgsl_math_for(int field) const {
 
 linon_status Context::read(const void* src, void* dst, size_t want, size_t bytes, const char* what) {
     if (!dst) return fail(LINON_ERR_ARGUMENT, std::string(what) + ": null destination");
+    if (!src) return fail(LINON_ERR_STATE, std::string(what) + ": no buffer (the last linon_resize failed)");
     if (bytes < want)
         return fail(LINON_ERR_ARGUMENT, std::string(what) + ": buffer too small (" + std::to_string(bytes) +
                                             " < " + std::to_string(want) + ")");
@@ -387,8 +398,9 @@ linon_status linon_resize(linon_context* ctx, uint32_t width, uint32_t height) {
     LN_CTX(c);
     if (width < 2 || height < 2 || width > 16384 || height > 16384)
         return c->fail(LINON_ERR_ARGUMENT, "resize: width and height must be in [2, 16384]");
-    (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+    LN_CHECK_HIP(c, hipSetDevice(c->device));
+    // renders may still run on a caller's stream: wait for the whole device before freeing
+    LN_CHECK_HIP(c, hipDeviceSynchronize());
     return c->alloc_images(width, height);
 }
 

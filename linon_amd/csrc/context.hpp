@@ -136,7 +136,9 @@ struct Context {
     std::vector<uint32_t> h_faces;
 
     linon_status fail(linon_status s, const std::string& msg);
+    // (Re)allocates the image buffers for w x h; on failure none is left and the size is 0.
     linon_status alloc_images(uint32_t w, uint32_t h);
+    linon_status alloc_images_at(uint32_t w, uint32_t h);
     linon_status alloc_samples();
     void free_images();
     void free_mesh();

@@ -409,6 +409,7 @@ linon_status Context::surrogate_eval(const float* states, const float* targets, 
 linon_status Context::surrogate_frame(hipStream_t s) {
     Surrogate& S = sg;
     if (!S.trained) return fail(LINON_ERR_STATE, "render_surrogate: no trained surrogate (linon_surrogate_train)");
+    if (!d_color) return fail(LINON_ERR_STATE, "render_surrogate: no image buffers (the last linon_resize failed)");
     const int W = (int)width, H = (int)height, npx = W * H;
     LS_CHECK(this, hipSetDevice(device));
     if (S.frame_px != npx) {
