@@ -200,7 +200,9 @@ void linon_destroy(linon_context* ctx);
 const char* linon_last_error(const linon_context* ctx);
 int linon_abi_version(void);
 
-/* MainView::resize_texture (src/main_view.rs:552-647). */
+/* MainView::resize_texture (src/main_view.rs:552-647).  Waits for all work on the device,
+ * then reallocates the image buffers.  If that fails, the context holds no images: renders and
+ * reads return LINON_ERR_STATE until a resize succeeds. */
 linon_status linon_resize(linon_context* ctx, uint32_t width, uint32_t height);
 /* MainView::update_camera (src/main_view.rs:490-494). */
 linon_status linon_set_camera(linon_context* ctx, const linon_camera* camera);
