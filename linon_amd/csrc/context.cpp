@@ -373,11 +373,13 @@ void linon_destroy(linon_context* ctx) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c) return;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    // renders may still run on a caller's stream: wait for the whole device before freeing
+    (void)hipDeviceSynchronize();
     c->free_images();
     c->free_mesh();
     c->release_jit();
     c->batch.release();
+    c->sg.release();
     if (c->d_samples) (void)hipFree(c->d_samples);
     if (c->d_counters) (void)hipFree(c->d_counters);
     if (c->d_outline) (void)hipFree(c->d_outline);

@@ -135,3 +135,30 @@ def test_surrogate_frame_against_plain_rk4(trained):
     if st["rejudged"]:
         assert np.isfinite(st["rejudge_err_max"])
         assert st["rejudge_err_q50"] <= st["rejudge_err_max"]
+
+
+def _device_free_bytes() -> int:
+    """Free memory of device 0 as the HIP runtime reports it (device-wide)."""
+    hip = C.CDLL("libamdhip64.so.7")
+    free, total = C.c_size_t(0), C.c_size_t(0)
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def test_closing_a_view_frees_its_surrogate():
+    """linon_destroy releases every device buffer of the context, the trained surrogate's
+    included (2^22 training states and labels, weights, optimizer moments, activations: several
+    hundred MB).  Three create / train / close cycles leave the device's free memory where it
+    was after the first."""
+    def cycle():
+        with L.MainView(64, 48) as view:
+            view.update_model(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
+            view.train_surrogate(L.SurrogateConfig.make("warp", train_rays=1 << 22, batch=8192, steps=5, h=0.01,
+                                                        max_steps=500))
+    print("[case] surrogate create/train/close cycles", flush=True)
+    cycle()   # the first use of each kernel and the runtime's own allocations
+    before = _device_free_bytes()
+    for _ in range(3):
+        cycle()
+    after = _device_free_bytes()
+    assert before - after < 128 << 20, (before, after)
