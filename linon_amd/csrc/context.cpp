@@ -43,6 +43,12 @@ void Context::free_images() {
 }
 
 void Context::free_mesh() {
+    // the captured batch graph holds these pointers by value: drop it (a successful upload
+    // captures it again), so that it never runs on freed buffers
+    if (batch.exec) (void)hipGraphExecDestroy(batch.exec);
+    if (batch.graph) (void)hipGraphDestroy(batch.graph);
+    batch.exec = nullptr;
+    batch.graph = nullptr;
     for (void** p : {(void**)&d_tris, (void**)&d_cell_start, (void**)&d_cell_faces, (void**)&d_dist}) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
@@ -279,7 +285,7 @@ linon_status Context::upload_mesh(const float* xyz, uint32_t nv, const uint32_t*
         num_faces = nf;
         have_mesh = true;
         grid_on_device = true;
-        return LINON_OK;
+        return batch.d_views ? batch_capture() : LINON_OK;
     }
     GridBuild g;
     std::string e;
@@ -313,7 +319,7 @@ linon_status Context::upload_mesh(const float* xyz, uint32_t nv, const uint32_t*
     num_faces = nf;
     have_mesh = true;
     grid_on_device = false;
-    return LINON_OK;
+    return batch.d_views ? batch_capture() : LINON_OK;  // a prepared batch renders the new mesh
 }
 
 }  // namespace linon
