@@ -91,8 +91,14 @@ linon_status Context::alloc_images_at(uint32_t w, uint32_t h) {
 linon_status Context::alloc_samples() {
     const size_t bytes = (size_t)8 * std::max(1, integ.samples_per_ray) * 32;
     if (d_samples && samples_bytes == bytes) return LINON_OK;
-    if (d_samples) (void)hipFree(d_samples);
+    if (d_samples) {
+        // renders may still run on a caller's stream: wait for the whole device before freeing
+        LN_CHECK_HIP(this, hipSetDevice(device));
+        LN_CHECK_HIP(this, hipDeviceSynchronize());
+        (void)hipFree(d_samples);
+    }
     d_samples = nullptr;
+    samples_bytes = 0;
     LN_CHECK_HIP(this, hipMalloc(&d_samples, bytes));
     LN_CHECK_HIP(this, hipMemset(d_samples, 0, bytes));
     samples_bytes = bytes;
@@ -179,6 +185,9 @@ linon_status Context::render_with(const linon_integrator& ig, hipStream_t s) {
     if (!have_mesh) return fail(LINON_ERR_STATE, "render: no mesh set (linon_set_mesh)");
     if (field_id < 0 && !jit_fn) return fail(LINON_ERR_STATE, "render: no field selected");
     if (!d_color) return fail(LINON_ERR_STATE, "render: no image buffers (the last linon_resize failed)");
+    if (ig.samples_per_ray > 0 && ig.sample_steps > 0 &&
+        (!d_samples || samples_bytes < (size_t)8 * ig.samples_per_ray * 32))
+        return fail(LINON_ERR_STATE, "render: no ray-sample buffer of that size (the last linon_set_integrator failed)");
     LN_CHECK_HIP(this, hipSetDevice(device));
     if (!s) s = stream;
     RaycastParams P;

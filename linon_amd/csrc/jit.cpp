@@ -102,7 +102,12 @@ void copy_out(const std::string& s, char* out, size_t cap) {
 }  // namespace
 
 void Context::release_jit() {
-    if (jit_module) (void)hipModuleUnload(jit_module);
+    if (jit_module) {
+        // a render on any stream may still be running the module's kernel
+        (void)hipSetDevice(device);
+        (void)hipDeviceSynchronize();
+        (void)hipModuleUnload(jit_module);
+    }
     jit_module = nullptr;
     jit_fn = nullptr;
     jit_body.clear();
