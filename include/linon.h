@@ -292,9 +292,10 @@ linon_status linon_set_batch_tuning(linon_context* ctx, const linon_batch_tuning
 linon_status linon_get_batch_tuning(const linon_context* ctx, linon_batch_tuning* tuning);
 /* Times candidate launch shapes on a copy of the current batch with every view scaled to
  * 1/`downscale` of its width and height (coordinate descent over waves per CU, waves per
- * block, steps per compaction round and refill threshold), times the three fastest once more
- * on the batch itself, keeps the fastest there and re-captures the batch with it.  best_ms:
- * the batch's own time with the chosen shape. */
+ * block, steps per compaction round and refill threshold), times the five fastest once more
+ * on the batch itself, sweeps waves per CU (16 and up) around the fastest there, keeps the
+ * best and re-captures the batch with it.  best_ms: the batch's own time with the chosen
+ * shape. */
 linon_status linon_autotune_batch(linon_context* ctx, int downscale, linon_batch_tuning* best, float* best_ms);
 
 /* Surrogate (no counterpart in the reference).  linon_surrogate_train generates the data set,

@@ -523,7 +523,7 @@ class MainView:
         self._check(self._lib.linon_set_batch_tuning(self._h, C.byref(t)))
 
     def autotune_batch(self, downscale: int = 4):
-        """Searches the launch shape on a 1/downscale proxy of the batch, confirms the three
+        """Searches the launch shape on a 1/downscale proxy of the batch, confirms the five
         fastest on the batch itself; returns (tuning, the batch's time in ms with it)."""
         t = BatchTuning()
         ms = C.c_float(0)

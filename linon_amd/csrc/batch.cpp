@@ -298,7 +298,7 @@ linon_status linon_autotune_batch(linon_context* ctx, int downscale, linon_batch
         }
     }
     P.release();
-    // the proxy ranks shapes only roughly at full size: the three fastest on the proxy are
+    // the proxy ranks shapes only roughly at full size: the five fastest on the proxy are
     // timed once more on the batch itself, and the fastest there is kept
     if (st == LINON_OK && !seen.empty()) {
         std::stable_sort(seen.begin(), seen.end(),
@@ -312,7 +312,7 @@ linon_status linon_autotune_batch(linon_context* ctx, int downscale, linon_batch
             bool dup = false;
             for (const auto& f : finalists) dup = dup || same(f, e.second);
             if (!dup) finalists.push_back(e.second);
-            if (finalists.size() == 3) break;
+            if (finalists.size() == 5) break;
         }
         float best_full = 0.0f;
         for (size_t i = 0; i < finalists.size() && st == LINON_OK; ++i) {
@@ -321,6 +321,22 @@ linon_status linon_autotune_batch(linon_context* ctx, int downscale, linon_batch
             if (st == LINON_OK && (i == 0 || ms < best_full)) {
                 best_full = ms;
                 cur = finalists[i];
+                cur_ms = ms;
+            }
+        }
+        // waves per CU depends on the amount of work: the proxy has 1/downscale^2 of the rays,
+        // where its tail favours fewer persistent waves than the batch wants, so that knob is
+        // also swept on the batch itself around the pick
+        const linon_batch_tuning pick = cur;
+        for (int v : wpc) {
+            if (st != LINON_OK) break;
+            if (v == pick.waves_per_cu || v < 16) continue;
+            linon_batch_tuning T = pick;
+            T.waves_per_cu = v;
+            float ms = 0.0f;
+            st = time_on(c->batch, T, 1, ms);
+            if (st == LINON_OK && ms < cur_ms) {
+                cur = T;
                 cur_ms = ms;
             }
         }
