@@ -79,7 +79,12 @@ struct FieldIdentity {  // src/main_view.wgsl:148
 };
 struct FieldSwirl {  // rotateY(v0, 0.5 * length(p))
     __device__ __forceinline__ vec3 operator()(vec3, vec3 p, vec3 v0, vec3, float) const {
+#ifdef LINON_FAST_MATH
+        // the angle's factor 0.5 joins the turns conversion (exact for |p| >= 2^-125)
+        return rotateY_turns(v0, length(p) * (0.5f * INV_2PI));
+#else
         return rotateY(v0, 0.5f * length(p));
+#endif
     }
 };
 struct FieldLens {  // v - 0.3 * p / max(|p|^3, 1e-3)

@@ -236,6 +236,18 @@ __device__ __forceinline__ vec3 rotateY(vec3 v, float phi) {
     sin_cos(phi, &s, &c);
     return {c * v.x + s * v.z, v.y, -s * v.x + c * v.z};
 }
+#ifdef LINON_FAST_MATH
+// rotateY(v, 2 pi turns) with the hardware sine and cosine, which take their argument in
+// turns: rotateY(v, phi) computes them of RN(phi * RN(1 / (2 pi))), so a caller whose angle is
+// a power of two times x (swirl: 0.5 |p|) passes RN(x * (0.5 RN(1 / (2 pi)))), the same value
+// wherever that scaling of x is exact, one multiplication fewer.
+__device__ __forceinline__ vec3 rotateY_turns(vec3 v, float turns) {
+    const float s = __builtin_amdgcn_sinf(turns);
+    const float c = __builtin_amdgcn_cosf(turns);
+    return {c * v.x + s * v.z, v.y, -s * v.x + c * v.z};
+}
+constexpr float INV_2PI = 0.15915494f;  // RN(1 / (2 pi)), the factor of the hardware sin/cos
+#endif
 __device__ __forceinline__ vec3 rotateZ(vec3 v, float phi) {
     float s, c;
     sin_cos(phi, &s, &c);
