@@ -207,6 +207,97 @@ def test_mirage_and_hot_air_temperature_regimes():
     assert abs(r[0]) < abs(v[0]) or abs(r[1]) != abs(v[1])
 
 
+# Temperature laws of the refracting fields, pinned at points where the law has a closed-form
+# value.  Each field evaluation across two such points must be Snell refraction at the index
+# ratio n(T(p_prev)) / n(T(p)) (n is pinned by its table above) about the field's normal.  The
+# refracted direction below is built from the angle law itself (tangential part scaled by eta,
+# unit length, same side as v), not from refraction()'s formula.
+def _sig(x):
+    return 1.0 / (1.0 + math.exp(-x))
+
+
+def _snell(v, n, eta):
+    v = np.asarray(v, np.float64)
+    n = np.asarray(n, np.float64) / np.linalg.norm(n)
+    vn = float(np.dot(v, n))
+    tang = eta * (v - vn * n)
+    s2 = float(np.dot(tang, tang))
+    assert s2 < 1.0, "pin cases avoid total internal reflection"
+    return tang + math.copysign(math.sqrt(1.0 - s2), vn) * n
+
+
+def _unit(*x):
+    x = np.asarray(x, np.float64)
+    return x / np.linalg.norm(x)
+
+
+HOT_AIR_C = np.array([0.0, -0.5, 0.0])          # functions of the benchmark's hot-air field
+MIRAGE_SPH_C = np.array([-0.5, 0.5, -0.5])      # src/functions.rs:39,70
+PLANE_Y = 0.1                                   # src/functions.rs:56,87
+
+
+# (field id, p_prev, p, T(p_prev), T(p), normal) with the temperatures in closed form:
+#  hot air  T = 20 + 80 sigmoid(8 (0.5 - |p - c|)): 60 at |p - c| = 0.5, 20 + 80 s(4) at c,
+#           20 + 80 s(2) at 0.25, 20 + 80 s(-4) at 1.0
+#  spherical mirage (linear) part = min(1, d / 0.25): 200 at the centre, 107.5 at d = 0.125,
+#           153.75 at 0.0625, 15 from 0.25 on
+#  spherical sigmoid part = min(1, s(12 d / 0.25 - 6)): 107.5 at d = 0.125, 15 s(-6) + 200 (1 - s(-6))
+#           at the centre, 15 s(3) + 200 (1 - s(3)) at 0.1875
+#  plane mirages (y = 0.1, max_dist 0.01, 30 -> 15): linear 30 on the plane, 22.5 at 0.005 either
+#           side, 15 from 0.01 on; sigmoid 22.5 at 0.005, 15 s(-6) + 30 (1 - s(-6)) on the plane
+_u = _unit(0.3, 0.8, -0.52)
+_w = _unit(-0.6, 0.2, 0.77)
+TEMPERATURE_CASES = [
+    ("hot-air 60 -> 20+80s(2)", 19, HOT_AIR_C + 0.5 * _u, HOT_AIR_C + 0.25 * _w, 60.0, 20 + 80 * _sig(2.0),
+     _w),
+    ("hot-air centre -> 60", 19, HOT_AIR_C, HOT_AIR_C + 0.5 * _w, 20 + 80 * _sig(4.0), 60.0, _w),
+    ("hot-air 20+80s(-4) -> 60", 19, HOT_AIR_C + 1.0 * _u, HOT_AIR_C + 0.5 * _u, 20 + 80 * _sig(-4.0), 60.0, _u),
+    ("mirage-sph 200 -> 107.5", 1, MIRAGE_SPH_C, MIRAGE_SPH_C + 0.125 * _u, 200.0, 107.5, _u),
+    ("mirage-sph 153.75 -> 15", 1, MIRAGE_SPH_C + 0.0625 * _w, MIRAGE_SPH_C + 0.3 * _u, 153.75, 15.0, _u),
+    ("mirage-sph-sig centre -> 107.5", 3, MIRAGE_SPH_C, MIRAGE_SPH_C + 0.125 * _w,
+     15 * _sig(-6.0) + 200 * (1 - _sig(-6.0)), 107.5, _w),
+    ("mirage-sph-sig 0.1875 -> 107.5", 3, MIRAGE_SPH_C + 0.1875 * _u, MIRAGE_SPH_C + 0.125 * _u,
+     15 * _sig(3.0) + 200 * (1 - _sig(3.0)), 107.5, _u),
+    ("mirage-plane 30 -> 22.5", 2, np.array([0.2, PLANE_Y, -0.1]), np.array([0.2, PLANE_Y + 0.005, -0.1]), 30.0,
+     22.5, np.array([0.0, 1.0, 0.0])),
+    ("mirage-plane 22.5 (below) -> 15", 2, np.array([0.0, PLANE_Y - 0.005, 0.3]), np.array([0.0, PLANE_Y + 0.02, 0.3]),
+     22.5, 15.0, np.array([0.0, 1.0, 0.0])),
+    ("mirage-plane-sig plane -> 22.5", 4, np.array([0.1, PLANE_Y, 0.0]), np.array([0.1, PLANE_Y - 0.005, 0.0]),
+     15 * _sig(-6.0) + 30 * (1 - _sig(-6.0)), 22.5, np.array([0.0, 1.0, 0.0])),
+]
+
+
+@pytest.mark.parametrize("label,fid,p_prev,p,t_in,t_out,normal", TEMPERATURE_CASES,
+                         ids=[c[0] for c in TEMPERATURE_CASES])
+def test_refracting_fields_temperature_laws(label, fid, p_prev, p, t_in, t_out, normal):
+    eta = O.refraction_index(t_in) / O.refraction_index(t_out)
+    assert abs(eta - 1.0) > 1e-3, "the pin needs a real index step"
+    # directions at 0.2 and 0.5 rad from the normal, travelling along it and against it, each
+    # below total internal reflection (eta <= 1.4 here, so sin(0.5) eta < 0.7)
+    n = np.asarray(normal, np.float64)
+    tan = np.cross(n, [0.3, 0.5, 0.81])
+    tan /= np.linalg.norm(tan)
+    for theta in (0.2, 0.5):
+        for side in (1.0, -1.0):
+            v = side * math.cos(theta) * n + math.sin(theta) * tan
+            got = O.field(fid, p_prev, p, v, v, 0.0)
+            np.testing.assert_allclose(got, _snell(v, n, eta), atol=1e-12, err_msg=f"{label} theta={theta}")
+
+
+def test_temperature_pins_separate_plausible_mistakes():
+    """The temperature pairs above differ enough that a mistaken law (sigmoid argument negated,
+    slope 8 -> 4, centre moved by 0.1, hot and cold swapped) gives an index ratio whose
+    refraction differs by more than the pins' tolerance."""
+    v = _unit(0.1, -0.3, 1.0)
+    n = _w
+    good = _snell(v, n, O.refraction_index(20 + 80 * _sig(4.0)) / O.refraction_index(60.0))
+    for wrong_t_in in (20 + 80 * _sig(-4.0), 20 + 80 * _sig(2.0), 20 + 80 * _sig(8 * (0.5 - 0.1))):
+        bad = _snell(v, n, O.refraction_index(wrong_t_in) / O.refraction_index(60.0))
+        assert np.abs(bad - good).max() > 1e-4
+    swapped = _snell(v, n, O.refraction_index(60.0) / O.refraction_index(20 + 80 * _sig(4.0)))
+    assert np.abs(swapped - good).max() > 1e-4
+
+
 # ---------------------------------------------------------------------------------------
 # RK4 loop (src/main_view.wgsl:229-279)
 # ---------------------------------------------------------------------------------------
