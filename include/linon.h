@@ -82,7 +82,10 @@ typedef struct linon_integrator {
                                  (exponents buffer of the previous render's overlay) is >= 0.8,
                                  one per side/corner (sample_outline_rays, main_view.wgsl:381-468);
                                  0: around the mouse position (:469-472), the reference's build */
-    int32_t pad[2];
+    int32_t clip_exit;        /* escape mode, 1: a ray that leaves the sphere from inside ends on it, at the
+                                 crossing of its last step with the sphere, instead of at the first point
+                                 past it (a smooth function of the ray; the flow map's exit point) */
+    int32_t pad;
 } linon_integrator;
 
 /* Field ids: 1..9 follow PredefinedFunction (src/functions.rs:2-13); 0 (Custom) has no
@@ -280,9 +283,11 @@ linon_status linon_synchronize(linon_context* ctx);
 /* Batches (no counterpart in the reference, which renders one view per MainView::render).
  * linon_set_batch uploads the views, allocates their outputs and captures the batch (queue
  * resets + one persistent launch per field) into a hipGraph; linon_render_batch replays it.
- * The integration loop (mode, t_max, max_steps, adaptive, escape radius) is the context's
- * integrator; the scene is the context's mesh.  Sample rays and the overlay are not part of
- * a batch. */
+ * The integration loop (mode, t_max, max_steps, adaptive, escape radius, clip_exit) is the
+ * context's integrator, and the scene is the context's mesh and acceleration choice, each as
+ * set at the time of linon_render_batch: linon_set_integrator, linon_set_mesh, linon_set_accel
+ * and linon_set_math re-capture a prepared batch (after waiting for the device).  Sample rays
+ * and the overlay are not part of a batch. */
 linon_status linon_set_batch(linon_context* ctx, const linon_view* views, int num_views);
 linon_status linon_render_batch(linon_context* ctx, void* stream);
 linon_status linon_read_batch_view(linon_context* ctx, int view, uint8_t* rgba, size_t rgba_bytes, float* mapping,
@@ -309,6 +314,14 @@ linon_status linon_surrogate_train(linon_context* ctx, const linon_surrogate_con
 linon_status linon_render_surrogate(linon_context* ctx, void* stream);
 linon_status linon_get_surrogate_stats(linon_context* ctx, linon_surrogate_stats* out);
 
+/* Diagnostics (no counterpart in the reference): the flow map through the bounding sphere that
+ * the surrogate learns, from host entry states.  Each state (p, q) of `states` [n][6] starts a ray
+ * at p with v0 = current direction = q and integrates it without a mesh under `integrator`, which
+ * must be in escape mode (clip_exit as given), with the kernel build `math` selects for the field
+ * (linon_math); exits [n][6] receives the end point and the final direction, steps[n] (optional)
+ * the RK4 iterations.  Fields: swirl, lens, hot air, warp, identity, spherical sigmoid mirage. */
+linon_status linon_flowmap(int device, int field, int math, const linon_integrator* integrator, float field_weight,
+                           const float* states, int n, float* exits, int32_t* steps);
 /* Diagnostics: the reciprocal-based divisions of the IEEE build against the IEEE division, bit
  * for bit: the reciprocal for every mantissa at exponents -126..125 (both signs); the
  * shared-denominator division (lw::div3_rn) for every numerator mantissa at three exponents

@@ -128,17 +128,18 @@ class Integrator(C.Structure):
     _fields_ = [("h_initial", C.c_float), ("t_max", C.c_float), ("adaptive", C.c_int32),
                 ("max_steps", C.c_int32), ("mode", C.c_int32), ("escape_radius", C.c_float),
                 ("samples_per_ray", C.c_int32), ("sample_steps", C.c_int32), ("need_mapping", C.c_int32),
-                ("outline_rays", C.c_int32), ("pad", C.c_int32 * 2)]
+                ("outline_rays", C.c_int32), ("clip_exit", C.c_int32), ("pad", C.c_int32)]
 
     @classmethod
     def make(cls, h_initial=0.1, t_max=5.0, adaptive=True, max_steps=100000, mode=MODE_REFERENCE,
              escape_radius=2.0, samples_per_ray=100, sample_steps=5000, need_mapping=False,
-             outline_rays=False) -> "Integrator":
+             outline_rays=False, clip_exit=False) -> "Integrator":
         i = cls()
         i.h_initial, i.t_max, i.adaptive, i.max_steps = h_initial, t_max, int(adaptive), max_steps
         i.mode, i.escape_radius = mode, escape_radius
         i.samples_per_ray, i.sample_steps, i.need_mapping = samples_per_ray, sample_steps, int(need_mapping)
         i.outline_rays = int(outline_rays)
+        i.clip_exit = int(clip_exit)
         return i
 
 
@@ -272,6 +273,7 @@ SIGNATURES = {
     "linon_selftest_gemm": (C.c_int, [C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P,
                                       C.c_int, _P, _P]),
     "linon_reset_timings": (C.c_int, [_P]),
+    "linon_flowmap": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(Integrator), C.c_float, _P, C.c_int, _P, _P]),
 }
 
 
@@ -563,6 +565,21 @@ class MainView:
     def read_frame(self) -> Frame:
         return Frame(self.read_final(), self.read_ray_casting(), self.read_mapping(), self.read_hit_faces(),
                      self.read_steps(), self.read_lyapunov(), self.read_exponents(), self.counters())
+
+
+def flowmap(field, integrator: Integrator, states, field_weight: float = 1.0, math: int = 0, device: int = 0):
+    """Flow map through the bounding sphere on the GPU (linon_flowmap): (n, 6) entry states
+    (p, q) -> ((n, 6) end point and final direction, (n,) RK4 iterations)."""
+    lib = load_library()
+    fid = FIELDS[field] if isinstance(field, str) else int(field)
+    x = np.ascontiguousarray(states, np.float32).reshape(-1, 6)
+    out = np.zeros_like(x)
+    steps = np.zeros(len(x), np.int32)
+    st = lib.linon_flowmap(device, fid, math, C.byref(integrator), field_weight, _ptr(x), len(x), _ptr(out),
+                           _ptr(steps))
+    if st != LINON_OK:
+        raise LinonError(st, "linon_flowmap failed")
+    return out, steps
 
 
 def selftest_sqrt(device: int = 0) -> dict:

@@ -39,6 +39,7 @@ void BatchState::release() {
     }
     groups.clear();
     views.clear();
+    h_views.clear();
     view_px.clear();
     total_px = 0;
 }
@@ -99,6 +100,7 @@ linon_status Context::batch_prepare(BatchState& B, const linon_view* views, int 
         g.total_rays = ray;
         B.groups.push_back(g);
     }
+    B.h_views = hv;
     LB_CHECK(this, hipMalloc(&B.d_views, hv.size() * sizeof(BatchView)));
     LB_CHECK(this, hipMemcpy(B.d_views, hv.data(), hv.size() * sizeof(BatchView), hipMemcpyHostToDevice));
     LB_CHECK(this, hipMalloc(&B.d_queues, B.groups.size() * sizeof(unsigned long long)));
@@ -133,6 +135,25 @@ linon_status Context::batch_launch(BatchState& B, const linon_batch_tuning& T, h
                                                : launch_batch(g.field, P, blocks, 64u * (unsigned)wpb, s));
     }
     return LINON_OK;
+}
+
+// The integration loop of a prepared batch is the context's integrator: after
+// linon_set_integrator, every view's step parameters (keeping its own h and field weight) and
+// the batch's shared ones are rewritten, and the graph, which holds the latter by value, is
+// captured again.  Waits for the device first: a replay in flight reads the views.
+linon_status Context::batch_set_step() {
+    if (!batch.d_views) return LINON_OK;
+    LB_CHECK(this, hipSetDevice(device));
+    LB_CHECK(this, hipDeviceSynchronize());
+    fill_step(batch.step, integ, 1.0f);
+    for (BatchView& bv : batch.h_views) {
+        const float h = bv.step.h_initial, w = bv.step.field_weight;
+        fill_step(bv.step, integ, w);
+        bv.step.h_initial = h;
+    }
+    LB_CHECK(this, hipMemcpy(batch.d_views, batch.h_views.data(), batch.h_views.size() * sizeof(BatchView),
+                             hipMemcpyHostToDevice));
+    return batch_capture();
 }
 
 linon_status Context::batch_capture() {

@@ -133,6 +133,7 @@ void fill_step(StepParams& I, const linon_integrator& ig, float field_weight) {
     I.max_steps = ig.max_steps;
     I.mode = ig.mode;
     I.need_mapping = ig.need_mapping;
+    I.clip_exit = ig.clip_exit;
 }
 
 void fill_camera(CameraParams& C, const linon_camera& cam, uint32_t w, uint32_t h) {
@@ -235,7 +236,9 @@ linon_status Context::render_with(const linon_integrator& ig, hipStream_t s) {
     return LINON_OK;
 }
 
-bool Context::wgsl_math_for(int field) const {
+bool Context::wgsl_math_for(int field) const { return wgsl_math_default(math, field); }
+
+bool wgsl_math_default(int math, int field) {
     if (math == LINON_MATH_IEEE) return false;
     if (math == LINON_MATH_WGSL) return true;
     switch (field) {  // AUTO
@@ -358,7 +361,7 @@ linon_status linon_create(linon_context** out, int device, uint32_t width, uint3
     }
     // MainView::new defaults (src/main_view.rs:100,133-139; src/main_view.wgsl:227,241-244,287-293)
     c->settings = linon_settings{1.0f, {0.5f, 0.6f}, 0, 1, 50.0f};
-    c->integ = linon_integrator{0.1f, 5.0f, 1, 100000, LINON_MODE_REFERENCE, 2.0f, 100, 5000, 0, 0, {0, 0}};
+    c->integ = linon_integrator{0.1f, 5.0f, 1, 100000, LINON_MODE_REFERENCE, 2.0f, 100, 5000, 0, 0, 0, 0};
     c->field_id = LINON_FIELD_MIRAGE_SPHERICAL_SIGMOID;
     // arcball default (src/arcball.rs:33-46 + zoom(-1)): eye = centre + (0,0,2) looking down -z
     std::memset(&c->camera, 0, sizeof(c->camera));
@@ -465,11 +468,15 @@ linon_status linon_set_integrator(linon_context* ctx, const linon_integrator* ig
         return c->fail(LINON_ERR_ARGUMENT, "set_integrator: max_steps must be in [0, 1e8]");
     if (ig->mode != LINON_MODE_REFERENCE && ig->mode != LINON_MODE_ESCAPE)
         return c->fail(LINON_ERR_ARGUMENT, "set_integrator: unknown mode");
+    if (ig->clip_exit != 0 && ig->clip_exit != 1)
+        return c->fail(LINON_ERR_ARGUMENT, "set_integrator: clip_exit must be 0 or 1");
     if (ig->samples_per_ray < 0 || ig->samples_per_ray > 100000 || ig->sample_steps < 0 ||
         (ig->samples_per_ray > 0 && ig->sample_steps < ig->samples_per_ray))
         return c->fail(LINON_ERR_ARGUMENT, "set_integrator: need 0 <= samples_per_ray <= sample_steps");
     c->integ = *ig;
-    return c->alloc_samples();
+    linon_status st = c->alloc_samples();
+    if (st == LINON_OK) st = c->batch_set_step();
+    return st;
 }
 
 linon_status linon_get_integrator(const linon_context* ctx, linon_integrator* ig) {
@@ -487,10 +494,13 @@ linon_status linon_set_accel(linon_context* ctx, int accel, int grid_resolution)
     if (grid_resolution != 0 && (grid_resolution < 4 || grid_resolution > 1024))
         return c->fail(LINON_ERR_ARGUMENT, "set_accel: grid_resolution must be 0 (auto) or in [4, 1024]");
     bool rebuild = c->have_mesh && grid_resolution != c->grid_res_req;
+    const bool changed = accel != c->accel_req;
     c->accel_req = accel;
     c->grid_res_req = grid_resolution;
     if (rebuild) return c->upload_mesh(c->h_xyz.data(), (uint32_t)(c->h_xyz.size() / 3), c->h_faces.data(),
                                        (uint32_t)(c->h_faces.size() / 3));
+    // the captured batch holds the scene parameters (the accel choice among them) by value
+    if (changed && c->batch.exec) return c->batch_capture();
     return LINON_OK;
 }
 

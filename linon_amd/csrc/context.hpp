@@ -23,6 +23,7 @@ struct BatchState {
     std::vector<size_t> view_px;    // pixel offset of each view in the arenas (+ total)
     size_t total_px = 0;
     std::vector<BatchGroup> groups;
+    std::vector<BatchView> h_views; // host copy of d_views
     BatchView* d_views = nullptr;   // grouped by field
     unsigned long long* d_queues = nullptr;
     unsigned* d_color = nullptr;
@@ -152,6 +153,7 @@ struct Context {
     linon_status batch_prepare(BatchState& B, const linon_view* views, int n);
     linon_status batch_launch(BatchState& B, const linon_batch_tuning& T, hipStream_t s);
     linon_status batch_capture();
+    linon_status batch_set_step();  // a prepared batch takes the context's current integrator
 
     Surrogate sg;
     linon_status surrogate_alloc_params();
@@ -171,6 +173,8 @@ struct Context {
 };
 
 void fill_step(StepParams& I, const linon_integrator& ig, float field_weight);
+// The kernel build (WGSL accuracy or IEEE) that linon_math value `math` selects for a field.
+bool wgsl_math_default(int math, int field);
 void fill_camera(CameraParams& C, const linon_camera& cam, uint32_t w, uint32_t h);
 
 }  // namespace linon

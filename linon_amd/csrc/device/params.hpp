@@ -41,6 +41,8 @@ struct SceneParams {
 struct StepParams {
     float h_initial, t_max, field_weight, escape_r2;
     int adaptive, max_steps, mode, need_mapping;
+    int clip_exit;  // escape mode: end a ray that leaves the sphere from inside on the sphere
+    int pad;
 };
 
 // One pinhole view (src/main_view.wgsl:347-366): CameraUniform's vectors and the viewport.

@@ -426,6 +426,21 @@ __device__ __forceinline__ void outline_search_body(const float* exps, int W, in
     }
 }
 
+// End point of a ray whose last step [p0, p0 + s] left the sphere |x|^2 = r2: the crossing
+// p0 + u s, u the larger root of |s|^2 u^2 + 2 (p0 . s) u + (|p0|^2 - r2) = 0, taken in the form
+// without cancellation for the sign of p0 . s and clamped to [0, 1].  A step that started
+// outside the sphere (|p0|^2 > r2) keeps its end point p0 + s.
+__device__ __forceinline__ vec3 sphere_exit(vec3 p0, vec3 s, float r2) {
+    const float c = rn::sub(rn::dot(p0, p0), r2);
+    if (!(c <= 0.0f)) return rn::add(p0, s);
+    const float a = rn::dot(s, s);
+    const float b = rn::dot(p0, s);
+    const float root = lw::sqrt_rn(rn::sub(rn::mul(b, b), rn::mul(a, c)));
+    float u = b > 0.0f ? (-c) / rn::add(b, root) : rn::sub(root, b) / a;
+    u = fminf(fmaxf(u, 0.0f), 1.0f);
+    return rn::add(p0, rn::mul(u, s));
+}
+
 // One ray's integration state (src/main_view.wgsl:233-242), kept in registers.
 struct RayState {
     vec3 p, d, v0, last_v;
@@ -559,7 +574,10 @@ __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, co
         }
         if (escape) {
             if (r.face >= 0 && !I.need_mapping) return true;
-            if (rn::dot(r.p, r.p) > I.escape_r2 && rn::dot(r.p, r.d) > 0.0f) return true;
+            if (rn::dot(r.p, r.p) > I.escape_r2 && rn::dot(r.p, r.d) > 0.0f) {
+                if (I.clip_exit) r.p = sphere_exit(p, step, I.escape_r2);
+                return true;
+            }
         }
     }
     return ray_finished(r, I);

@@ -340,6 +340,7 @@ StepParams Context::surrogate_step_params() const {
     I.max_steps = sg.cfg.max_steps;
     I.mode = LINON_MODE_ESCAPE;
     I.need_mapping = 1;
+    I.clip_exit = 1;
     return I;
 }
 
@@ -604,6 +605,41 @@ linon_status linon_get_surrogate_stats(linon_context* ctx, linon_surrogate_stats
 }
 
 }  // extern "C"
+
+extern "C" linon_status linon_flowmap(int device, int field, int math, const linon_integrator* integrator,
+                                      float field_weight, const float* states, int n, float* exits, int32_t* steps) {
+    if (!integrator || !states || !exits || n < 1 || n > (1 << 26) || integrator->mode != LINON_MODE_ESCAPE ||
+        math < LINON_MATH_AUTO || math > LINON_MATH_WGSL || !(integrator->h_initial > 0.0f) ||
+        integrator->max_steps < 1 || (integrator->clip_exit != 0 && integrator->clip_exit != 1))
+        return LINON_ERR_ARGUMENT;
+    if (!linon::sg_field_supported(field)) return LINON_ERR_UNSUPPORTED;
+    if (hipSetDevice(device) != hipSuccess) return LINON_ERR_HIP;
+    float *d_in = nullptr, *d_out = nullptr;
+    int* d_steps = nullptr;
+    linon_status st = LINON_OK;
+    auto ok = [&](hipError_t e) {
+        if (e != hipSuccess) st = LINON_ERR_HIP;
+        return e == hipSuccess;
+    };
+    const size_t bytes = (size_t)n * 6 * sizeof(float);
+    if (ok(hipMalloc(&d_in, bytes)) && ok(hipMalloc(&d_out, bytes)) && ok(hipMalloc(&d_steps, (size_t)n * 4)) &&
+        ok(hipMemcpy(d_in, states, bytes, hipMemcpyHostToDevice)) && ok(hipMemset(d_steps, 0, (size_t)n * 4))) {
+        ls::FlowmapParams F;
+        std::memset(&F, 0, sizeof(F));
+        linon::fill_step(F.step, *integrator, field_weight);
+        F.states = d_in;
+        F.n = n;
+        F.exits = d_out;
+        F.steps = d_steps;
+        if (ok(linon::wgsl_math_default(math, field) ? linon_wgslmath::sg_flowmap(field, F, nullptr)
+                                                      : linon::sg_flowmap(field, F, nullptr)) &&
+            ok(hipDeviceSynchronize()) && ok(hipMemcpy(exits, d_out, bytes, hipMemcpyDeviceToHost)) && steps)
+            ok(hipMemcpy(steps, d_steps, (size_t)n * 4, hipMemcpyDeviceToHost));
+    }
+    for (void* p : {(void*)d_in, (void*)d_out, (void*)d_steps})
+        if (p) (void)hipFree(p);
+    return st;
+}
 
 extern "C" linon_status linon_selftest_gemm(int device, const uint16_t* A, const uint16_t* B, int M, int N, int K,
                                             int a_trans, int b_trans, int splits, const float* bias, int silu,

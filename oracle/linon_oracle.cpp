@@ -321,7 +321,8 @@ struct IntegratorIn {
     int32_t sample_steps;
     int32_t need_mapping;  // escape mode only: keep integrating after a hit (overlay wants the end point)
     int32_t outline_rays;  // layout only: outline positions are passed to sample_rays explicitly
-    int32_t pad[2];
+    int32_t clip_exit;     // escape mode: a ray leaving the sphere from inside ends on it
+    int32_t pad;
 };
 
 template <typename T>
@@ -374,6 +375,20 @@ struct RayOut {
     T end_dir[3];  // cur_dir at the end of the loop
 };
 
+// Escape mode with clip_exit: the crossing of the last step [p0, p0 + s] with the sphere
+// |x|^2 = r2 when the step started inside it, as the larger root u in [0, 1] of
+// |s|^2 u^2 + 2 (p0 . s) u + (|p0|^2 - r2) = 0 (no counterpart in the reference).
+template <typename T> V3<T> sphere_exit(V3<T> p0, V3<T> s, T r2) {
+    T c = dot(p0, p0) - r2;
+    if (!(c <= T(0))) return p0 + s;
+    T a = dot(s, s);
+    T b = dot(p0, s);
+    T root = t_sqrt(b * b - a * c);
+    T u = b > T(0) ? t_div(-c, b + root) : t_div(root - b, a);
+    u = std::fmin(std::fmax(u, T(0)), T(1));
+    return p0 + u * s;
+}
+
 // nonlinear_ray_color, main_view.wgsl:229-279 (mode 0), plus the escape termination.
 template <typename T>
 RayOut<T> trace(const Scene<T>& sc, int field, const IntegratorIn& ig, T field_weight,
@@ -425,6 +440,7 @@ RayOut<T> trace(const Scene<T>& sc, int field, const IntegratorIn& ig, T field_w
             col = ray_color(sc, cur_point, normalize(step_dir), length(step_dir), &tests);
             has_color = col.color[3] > T(0);
         }
+        const V3<T> prev_point = cur_point;
         cur_point = cur_point + step_dir;
         t = t + h;
         t_loop = t_loop + (float)h;
@@ -433,7 +449,10 @@ RayOut<T> trace(const Scene<T>& sc, int field, const IntegratorIn& ig, T field_w
         last_diff = diff;
         if (ig.mode == 1) {
             if (has_color && !ig.need_mapping) break;
-            if (dot(cur_point, cur_point) > r2 && dot(cur_point, cur_dir) > T(0)) break;
+            if (dot(cur_point, cur_point) > r2 && dot(cur_point, cur_dir) > T(0)) {
+                if (ig.clip_exit) cur_point = sphere_exit(prev_point, step_dir, r2);
+                break;
+            }
         }
     }
     out.mapping[0] = cur_point.x;

@@ -57,7 +57,7 @@ class IntegratorIn(C.Structure):
     _fields_ = [("h_initial", C.c_float), ("t_max", C.c_float), ("adaptive", C.c_int32),
                 ("max_steps", C.c_int32), ("mode", C.c_int32), ("escape_radius", C.c_float),
                 ("samples_per_ray", C.c_int32), ("sample_steps", C.c_int32), ("need_mapping", C.c_int32),
-                ("outline_rays", C.c_int32), ("pad", C.c_int32 * 2)]
+                ("outline_rays", C.c_int32), ("clip_exit", C.c_int32), ("pad", C.c_int32)]
 
 
 _lib = None
@@ -182,11 +182,12 @@ def settings_in(field_weight=1.0, mouse_pos=(0.5, 0.6), overlay_mode=0, delta=1,
 
 
 def integrator_in(h_initial=0.1, t_max=5.0, adaptive=True, max_steps=100000, mode=0, escape_radius=2.0,
-                  samples_per_ray=100, sample_steps=5000, need_mapping=False) -> IntegratorIn:
+                  samples_per_ray=100, sample_steps=5000, need_mapping=False, clip_exit=False) -> IntegratorIn:
     i = IntegratorIn()
     i.h_initial, i.t_max, i.adaptive, i.max_steps, i.mode = h_initial, t_max, int(adaptive), max_steps, mode
     i.escape_radius, i.samples_per_ray, i.sample_steps = escape_radius, samples_per_ray, sample_steps
     i.need_mapping = int(need_mapping)
+    i.clip_exit = int(clip_exit)
     return i
 
 

@@ -8,6 +8,8 @@ import pytest
 import linon_amd as L
 from linon_amd import configs, scenes
 
+from gpu_common import compare, run_oracles
+
 pytestmark = pytest.mark.gpu
 
 
@@ -159,3 +161,60 @@ def test_mesh_change_recaptures_the_batch():
         assert np.array_equal(g.hit_faces, r.hit_faces)
         assert np.array_equal(g.mapping, r.mapping)
     assert any(not np.array_equal(g.hit_faces, a.hit_faces) for g, a in zip(got, first))
+
+
+def test_batch_follows_integrator_and_accel_changes(setup):
+    """linon_set_integrator and linon_set_accel after linon_set_batch reach the captured batch:
+    a smaller max_steps caps every view's iterations, and the brute-force loop renders the same
+    frames as the grid."""
+    view, views = setup
+    view.set_integrator(configs.batch_integrator(max_steps=3000))
+    view.set_batch(views)
+    view.render_batch()
+    full = [view.read_batch_view(i) for i in range(len(views))]
+    assert max(int(g.steps.max()) for g in full) > 40
+    print("[case] batch after set_integrator(max_steps=40)", flush=True)
+    view.set_integrator(configs.batch_integrator(max_steps=40))
+    view.render_batch()
+    capped = [view.read_batch_view(i) for i in range(len(views))]
+    for g, f in zip(capped, full):
+        assert g.steps.max() <= 40
+        assert np.array_equal(g.steps, np.minimum(f.steps, 40))
+    print("[case] batch after set_accel(brute)", flush=True)
+    view.set_accel(L.ACCEL_BRUTE)
+    view.render_batch()
+    c_brute = view.counters()
+    brute = [view.read_batch_view(i) for i in range(len(views))]
+    view.set_accel(L.ACCEL_AUTO)
+    view.render_batch()
+    c_grid = view.counters()
+    for b, g in zip(brute, capped):
+        assert np.array_equal(b.hit_faces, g.hit_faces) and np.array_equal(b.mapping, g.mapping)
+    assert c_brute["triangle_tests"] > c_grid["triangle_tests"]
+    view.set_integrator(configs.batch_integrator(max_steps=3000))
+
+
+def test_batch_views_against_the_oracle():
+    """Batch views (mixed fields, step sizes and resolutions; escape mode, first-hit stop)
+    compared with the oracle's escape mode directly, not only with the plain kernel: hit face,
+    end point and iteration count per view."""
+    v, f = scenes.benchmark_scene(level=2)
+    views = configs.batch_views(num_views=5, seed=19, scale=32, enhance=False)   # 32x32 and 60x33, 4 fields
+    with L.MainView(32, 32) as view:
+        view.update_model(v, f)
+        view.set_integrator(configs.batch_integrator(max_steps=3000))
+        print(f"[case] batch of {len(views)} views against the oracle", flush=True)
+        view.set_batch(views)
+        view.render_batch()
+        got = [view.read_batch_view(i) for i in range(len(views))]
+    fields = set()
+    for i, bv in enumerate(views):
+        ig = configs.batch_integrator(max_steps=3000)
+        ig.h_initial = bv.h_initial
+        ref, ref64 = run_oracles(bv.camera, L.Settings.make(field_weight=bv.field_weight), ig, bv.field, v, f,
+                                 bv.width, bv.height)
+        info = compare(got[i], ref, ref64, f"batch-view-{i}-field{bv.field}-h{bv.h_initial}-{bv.width}x{bv.height}")
+        assert info["hit_ok"] and info["exit_pass"] and info["q99_ok"], info
+        assert (got[i].steps == ref["steps"]).mean() >= 0.99
+        fields.add(bv.field)
+    assert len(fields) == 4

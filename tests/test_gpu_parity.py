@@ -78,6 +78,38 @@ def test_benchmark_scene_small(view, field):
     _case(view, 48, 32, wl.camera(), wl.settings(), wl.integrator(), field, v, f, f"bench-scene-{field}")
 
 
+@pytest.mark.parametrize("clip_exit", [0, 1])
+@pytest.mark.parametrize("need_mapping", [0, 1])
+@pytest.mark.parametrize("field", ["swirl", "lens", "hot_air", "warp"])
+def test_escape_mode(view, field, need_mapping, clip_exit):
+    """The escape mode that batches and the surrogate's RK4 legs integrate with (a ray stops when
+    it leaves the radius-2 sphere moving outward, at max_steps, or at its first hit unless
+    need_mapping; clip_exit ends it on the sphere) against the oracle's escape mode, on the
+    benchmark scene: hit face, end point, and the iteration count."""
+    wl = configs.OVERLAY_1080P
+    v, f = scenes.benchmark_scene(level=2)
+    ig = L.Integrator.make(h_initial=0.005, t_max=1e30, adaptive=False, max_steps=4000, mode=L.MODE_ESCAPE,
+                           escape_radius=2.0, samples_per_ray=0, sample_steps=0, need_mapping=bool(need_mapping),
+                           clip_exit=bool(clip_exit))
+    gpu, ref = _case(view, 48, 32, wl.camera(), wl.settings(), ig, field, v, f,
+                     f"escape-{field}-map{need_mapping}-clip{clip_exit}")
+    # the iteration count is decided by the escape test |p| > 2 on the f32 path; rays that end
+    # within rounding of the sphere may take one step more or less
+    assert (gpu.steps == ref["steps"]).mean() >= 0.99
+    hit = ref["hit"] >= 0
+    if not need_mapping:   # a hit ends the ray: its iteration count is the hit's step
+        assert hit.any()
+    if clip_exit:  # rays that crossed the sphere from inside (no hit, or mapping kept) end on it
+        left = (ref["steps"] < 4000) & (~hit | bool(need_mapping))
+        on = np.abs(np.linalg.norm(gpu.mapping[..., :3].astype(np.float64), axis=-1) - 2.0) <= 1e-5
+        on_ref = np.abs(np.linalg.norm(ref["mapping"][..., :3].astype(np.float64), axis=-1) - 2.0) <= 1e-5
+        # rays that never entered the sphere (swirl turns most camera rays away from it) end
+        # outside it; lens captures many rays near its singularity
+        if left.any():
+            assert (on == on_ref)[left].mean() >= 0.99
+        assert on.any() or field == "lens"
+
+
 @pytest.mark.parametrize("fid", [1, 2, 3, 4, 5, 6, 7, 8, 9])
 def test_predefined_fields(view, fid):
     """Every body of PredefinedFunction (src/functions.rs:33-128) on the start-up scene."""

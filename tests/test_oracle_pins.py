@@ -384,6 +384,48 @@ def test_escape_mode_stops_outside_the_sphere():
     assert end[2] < -2.0 and end[2] > -2.0 - 0.0101
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_clip_exit_is_the_sphere_crossing(precision):
+    """With clip_exit, a straight ray (identity field) ends where the line leaves the sphere:
+    o + t d with |o + t d| = R, t the larger root of t^2 + 2 (o.d) t + |o|^2 - R^2 = 0."""
+    rng = np.random.default_rng(11)
+    R = 2.0
+    tol = 1e-12 if precision == "f64" else 2e-5   # f32: p += d h rounds once per step
+    for _ in range(200):
+        o = rng.normal(size=3)
+        o *= rng.uniform(0.0, 1.9) / np.linalg.norm(o)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        h = float(rng.choice([0.01, 0.037, 0.1]))
+        ig = _ig(h_initial=h, t_max=1e30, mode=1, escape_radius=R, clip_exit=True)
+        _, end, _ = O.trace(FIELDS["identity"], ig, o, d, precision=precision)
+        b = float(np.dot(o, d))
+        t = -b + math.sqrt(b * b - (float(np.dot(o, o)) - R * R))
+        np.testing.assert_allclose(end, o + t * d, atol=tol)
+        # without the clip the ray ends at the first step point past the sphere
+        _, end_raw, _ = O.trace(FIELDS["identity"], _ig(h_initial=h, t_max=1e30, mode=1, escape_radius=R), o, d,
+                                precision=precision)
+        assert np.linalg.norm(end_raw) > R and np.linalg.norm(end_raw - end) <= h * (1 + 1e-5)
+
+
+def test_clip_exit_on_a_curved_path():
+    """Swirl bends the path; the clipped end lies on the sphere, on the last step's segment."""
+    ig = _ig(h_initial=0.05, t_max=1e30, mode=1, escape_radius=2.0, clip_exit=True)
+    ig_raw = _ig(h_initial=0.05, t_max=1e30, mode=1, escape_radius=2.0)
+    for o, d in (((0.3, 0.2, 1.9), (-0.1, 0.05, -1.0)), ((1.5, -0.4, 0.2), (-1.0, 0.3, 0.1))):
+        dd = np.asarray(d) / np.linalg.norm(d)
+        n, end, _ = O.trace(FIELDS["swirl"], ig, o, dd)
+        n_raw, end_raw, _ = O.trace(FIELDS["swirl"], ig_raw, o, dd)
+        _, prev, _ = O.trace(FIELDS["swirl"], _ig(h_initial=0.05, t_max=1e30, max_steps=n_raw - 1), o, dd)
+        assert n == n_raw
+        assert np.linalg.norm(end) == pytest.approx(2.0, abs=1e-12)
+        # collinear with the last step and between its ends
+        seg = end_raw - prev
+        u = float(np.dot(end - prev, seg) / np.dot(seg, seg))
+        assert 0.0 <= u <= 1.0
+        np.testing.assert_allclose(prev + u * seg, end, atol=1e-12)
+
+
 # ---------------------------------------------------------------------------------------
 # Lyapunov overlay (src/overlay.wgsl:4-146)
 # ---------------------------------------------------------------------------------------
