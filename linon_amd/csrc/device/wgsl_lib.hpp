@@ -198,11 +198,22 @@ __device__ __forceinline__ int clamp(int e, int lo, int hi) { return min(max(e, 
 __device__ __forceinline__ unsigned min(unsigned a, unsigned b) { return a < b ? a : b; }
 __device__ __forceinline__ unsigned max(unsigned a, unsigned b) { return a > b ? a : b; }
 __device__ __forceinline__ unsigned clamp(unsigned e, unsigned lo, unsigned hi) { return min(max(e, lo), hi); }
-// WGSL % : truncated remainder for floats (a - b * trunc(a / b)), C remainder for integers
+// WGSL % : truncated remainder for floats (a - b * trunc(a / b)).  Integer / and % are total
+// in WGSL: where C++ leaves x / 0 and INT_MIN / -1 undefined, WGSL divides by 1 instead, so
+// x / 0 = x, INT_MIN / -1 = INT_MIN, and the remainder of both cases is 0.
 __device__ __forceinline__ float wmod(float a, float b) { return a - b * __builtin_truncf(a / b); }
 __device__ __forceinline__ vec3 wmod(vec3 a, vec3 b) { return {wmod(a.x, b.x), wmod(a.y, b.y), wmod(a.z, b.z)}; }
-__device__ __forceinline__ int wmod(int a, int b) { return b == 0 ? a : a % b; }
-__device__ __forceinline__ unsigned wmod(unsigned a, unsigned b) { return b == 0u ? a : a % b; }
+__device__ __forceinline__ int wdivisor(int a, int b) { return (b == 0 || (a == (-2147483647 - 1) && b == -1)) ? 1 : b; }
+__device__ __forceinline__ int wdiv(int a, int b) { return a / wdivisor(a, b); }
+__device__ __forceinline__ int wmod(int a, int b) {
+    const int d = wdivisor(a, b);
+    return a - (a / d) * d;
+}
+__device__ __forceinline__ unsigned wdiv(unsigned a, unsigned b) { return a / (b == 0u ? 1u : b); }
+__device__ __forceinline__ unsigned wmod(unsigned a, unsigned b) {
+    const unsigned d = b == 0u ? 1u : b;
+    return a - (a / d) * d;
+}
 template <int X, int Y, int Z>
 __device__ __forceinline__ vec3 swizzle3(vec3 v) {
     const float c[3] = {v.x, v.y, v.z};

@@ -563,6 +563,8 @@ class Parser {
             rt = it;
         }
         if (op == "%") return {"lw::wmod(" + a.code + ", " + b.code + ")", rt};
+        if (op == "/" && (rt == Ty::I || rt == Ty::U || rt == Ty::AI))  // WGSL's total integer division
+            return {"lw::wdiv(" + a.code + ", " + b.code + ")", rt};
         return {"(" + a.code + " " + op + " " + b.code + ")", rt};
     }
 

@@ -365,3 +365,22 @@ def test_square_root_is_ieee():
     print(res, flush=True)
     assert res["comparisons"] == 1 << 32
     assert res["mismatches"] == 0, res
+
+
+def test_wgsl_integer_division_is_total(view):
+    """A WGSL body's integer / and % follow WGSL, not C++: x / 0 = x, x % 0 = 0,
+    INT_MIN / -1 = INT_MIN, INT_MIN % -1 = 0 (i32 and u32).  The body returns a constant
+    velocity built from those results, so each ray moves in a straight line at that velocity and
+    the end point reveals it: V = (1, 0.5, 2)."""
+    body = ("let z = i32(p.x * 0.0);\n"
+            "let m = -2147483647 - 1 + z;\n"
+            "let n = z - 1;\n"
+            "let u = u32(z);\n"
+            "return vec3<f32>(f32(7 / z) / 7.0, f32(7 % z) + f32(m % n) + f32(9u % u) + 0.5,\n"
+            "                 f32(m / n) / -2147483648.0 + f32(9u / u) / 9.0);")
+    view.resize_texture(16, 8)
+    ig = L.Integrator.make(h_initial=0.1, adaptive=False, samples_per_ray=0, sample_steps=0)
+    cam = L.Camera.look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), fov_deg=60.0)
+    fr = run_gpu(view, cam, L.Settings.make(), ig, body, np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
+    v = (fr.mapping[..., :3].astype(np.float64) - [0.0, 0.0, 3.0]) / (fr.steps[..., None] * 0.1)
+    np.testing.assert_allclose(v, np.broadcast_to([1.0, 0.5, 2.0], v.shape), atol=1e-4)

@@ -127,3 +127,12 @@ def test_reference_predefined_bodies_translate_and_compile():
         assert "return" in tr(b)
     for b in (bodies[2], bodies[7]):
         L.wgsl_compile_check(b)
+
+
+def test_integer_division_is_wgsls_total_division():
+    """WGSL integer / and % never trap: x / 0 = x, INT_MIN / -1 = INT_MIN, and both remainders 0
+    (C++ leaves those cases undefined), so integer division goes through lw::wdiv / lw::wmod."""
+    out = tr("let a: i32 = i32(t); let b = 7 / a; let c: u32 = u32(t); let d = 5u / c; let e = a % 3; "
+             "let g = 2.0 / t; return v * f32(b) * f32(d) * f32(e) * g;")
+    assert "lw::wdiv(7, a)" in out and "lw::wdiv(5u, c)" in out and "lw::wmod(a, 3)" in out
+    assert "(2.0f / t)" in out
