@@ -239,7 +239,7 @@ def main() -> int:
     dom = max(per_field, key=lambda n: per_field[n]["raycast_ms"])
     d = per_field[dom]
     kname = {"swirl": "FieldSwirl", "lens": "FieldLens", "hot_air": "FieldHotAir", "warp": "FieldWarp"}[dom]
-    roofline = {"bound": "mfma", "kernel": f"void linon::raycast_kernel<lw::{kname}>(RaycastParams)",
+    roofline = {"bound": "valu", "kernel": f"void linon::raycast_kernel<lw::{kname}>(RaycastParams)",
                 "note": "compute-bound on FP32 vector ALUs (no matrix cores); peak = MI355X FP32 157.3 TFLOP/s",
                 "achieved": d["tflops"],
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": d["tflops"] / PEAK_FP32_TFLOPS,
