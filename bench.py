@@ -174,7 +174,9 @@ def main() -> int:
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--batch-replays", type=int, default=2)
     ap.add_argument("--skip-surrogate", action="store_true")
-    ap.add_argument("--surrogate-steps", type=int, default=1500)
+    ap.add_argument("--surrogate-steps", type=int, default=configs.SURROGATE_STEPS)
+    ap.add_argument("--surrogate-train", action="store_true",
+                    help="train the surrogates in this run even where saved weights match")
     ap.add_argument("--surrogate-fields", default="swirl,lens,hot_air,warp")
     ap.add_argument("--skip-stress", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -289,14 +291,15 @@ def main() -> int:
         extra["batched_views"] = bench_batch(verts, faces, a.batch_replays)
     if not a.skip_surrogate:
         log("surrogate ...")
-        extra["surrogate"] = bench_surrogate(a.surrogate_steps, a.surrogate_fields.split(","))
+        extra["surrogate"] = bench_surrogate(a.surrogate_steps, a.surrogate_fields.split(","),
+                                             train=a.surrogate_train)
     if not a.skip_stress:
         log("4K stress ...")
         extra["stress_4k"] = bench_stress()
         if not a.skip_surrogate:
             log("4K stress surrogate ...")
             extra["stress_4k"]["surrogate"] = bench_surrogate(a.surrogate_steps, configs.STRESS_4K.fields,
-                                                              configs.STRESS_4K)
+                                                              configs.STRESS_4K, train=a.surrogate_train)
     out["other_workloads"] = extra
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
@@ -363,6 +366,8 @@ def bench_stress() -> dict:
         view.set_field(name)
         view.render()
         c = view.counters()
+        # the timed frame's outline search reads this frame's exponents
+        searched_hi = int((view.read_exponents() >= 0.8).sum())
         view.reset_timings()
         view.render()
         rms, pms = view.timings()
@@ -372,8 +377,10 @@ def bench_stress() -> dict:
                                   "triangle_tests": c["triangle_tests"],
                                   "gsteps_per_s": c["rk4_steps"] / (r_ms * 1e-3) / 1e9,
                                   "tflops": fl / (r_ms * 1e-3) / 1e12,
-                                  # outline searches (of 8) that found a highlighted texel
-                                  "outline_rays_found": int(view.read_outline()[:, 2].sum())}
+                                  # outline searches (of 8) that found a highlighted texel, and the
+                                  # highlighted texels (exponent >= 0.8) in the buffer they searched
+                                  "outline_rays_found": int(view.read_outline()[:, 2].sum()),
+                                  "searched_exponents_ge_0.8": searched_hi}
         log(f"stress {name}: {rms[0]:.1f} ms")
     # both fields' frames captured into one HIP graph on a caller's stream and replayed (the
     # graph keeps each capture's field; the frames share the output buffers)
@@ -429,23 +436,29 @@ def bench_batch(verts, faces, replays: int) -> dict:
             "tuning": tuning.as_dict(), "autotune_full_size_ms": tuned_ms, "autotune_seconds": tune_s}
 
 
-def bench_surrogate(steps: int, fields, wl=configs.OVERLAY_1080P) -> dict:
-    """Per field: train the flow-map surrogate (2^24 uniform entry states labelled by the FP32
-    RK4 kernel, batch 65536, Adam lr 1e-3), then render the workload's view (1080p by default)
-    without the mesh (mapping + overlay) twice: plain RK4 (escape mode) and surrogate +
-    fallback; report speed-up and the exit-point error of every pixel against the plain frame."""
+def bench_surrogate(steps: int, fields, wl=configs.OVERLAY_1080P, train: bool = False) -> dict:
+    """Per field: the flow-map surrogate of the workload's step h (configs.surrogate_config; the
+    saved weights under linon_amd/surrogates/ when they match it and `train` is off, else trained
+    here on 2^24 uniform entry states labelled by the FP32 RK4 kernel), then the workload's view
+    without the mesh (mapping + overlay), rendered twice: plain RK4 (escape mode, exits on the
+    sphere) and surrogate + fallback.  Reports the speed-up and, over the texels that took the
+    surrogate's exit, the fraction within the 1e-3 tolerance of the plain frame."""
     out = {}
     for field in fields:
         view = L.MainView(wl.width, wl.height)
         view.update_model(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
         view.update_camera(wl.camera())
         view.update_settings(wl.settings())
-        cfg = L.SurrogateConfig.make(field, train_rays=1 << 24, batch=65536, steps=steps, lr=1e-3, h=wl.h,
-                                     max_steps=wl.max_steps, tolerance=1e-3, conf_threshold=0.9)
-        tr = view.train_surrogate(cfg)
+        cfg = configs.surrogate_config(field, wl, steps=steps)
+        saved = configs.saved_surrogate(field, wl)
+        if saved is not None and not train:
+            view.load_surrogate(saved)
+            tr = dict(view.surrogate_stats(), source=f"saved weights ({saved.name})")
+        else:
+            tr = dict(view.train_surrogate(cfg), source="trained in this run")
         ig = L.Integrator.make(h_initial=wl.h, t_max=1e30, adaptive=False, max_steps=wl.max_steps,
                                mode=L.MODE_ESCAPE, escape_radius=2.0, samples_per_ray=0, sample_steps=0,
-                               need_mapping=True)
+                               need_mapping=True, clip_exit=True)
         view.set_integrator(ig)
         view.set_field(field)
         view.render()
@@ -457,23 +470,36 @@ def bench_surrogate(steps: int, fields, wl=configs.OVERLAY_1080P) -> dict:
         view.render_surrogate()
         st = view.surrogate_stats()
         sur = view.read_mapping()
-        err = np.linalg.norm(sur[..., :3].astype(np.float64) - plain[..., :3], axis=-1)
-        out[field] = {"train": {k: tr[k] for k in ("train_loss", "label_seconds", "train_seconds", "holdout_err_q50",
-                                                  "holdout_err_q99", "holdout_frac_within_tol")},
+        acc = view.read_surrogate_accepted()
+        a = sur[..., :3].astype(np.float64)[acc]
+        b = plain[..., :3].astype(np.float64)[acc]
+        rel = (np.abs(a - b) / np.maximum(1.0, np.abs(b))).max(axis=-1) if acc.any() else np.zeros(0)
+        out[field] = {"train": {k: tr.get(k) for k in ("source", "train_loss", "label_seconds", "train_seconds",
+                                                      "holdout_err_q50", "holdout_err_q99",
+                                                      "holdout_frac_within_tol")},
+                      "coarse_h": cfg.coarse_h, "tolerance": cfg.tolerance,
                       "plain_frame_ms": plain_ms, "surrogate_frame_ms": st["frame_ms"],
                       "speedup": plain_ms / st["frame_ms"] if st["frame_ms"] > 0 else None,
-                      "frac_pixels_within_1e-3": float((err <= 1e-3).mean()),
+                      # exit-point error (max over x, y, z of |d| / max(1, |plain|)) of the texels
+                      # that took the surrogate's exit, against the plain frame
+                      "accepted_within_tol": float((rel <= cfg.tolerance).mean()) if acc.any() else None,
+                      "accepted_err_q50_q99_max": ([float(np.quantile(rel, q)) for q in (0.5, 0.99, 1.0)]
+                                                   if acc.any() else None),
+                      "nothing_accepted": not bool(acc.any()),
                       "rays": int(st["frame_rays"]), "entered": int(st["entered"]), "accepted": int(st["accepted"]),
+                      "accepted_frac_of_entered": (int(st["accepted"]) / int(st["entered"])
+                                                   if st["entered"] else None),
                       "conf_fallback": int(st["conf_fallback"]), "lyapunov_fallback": int(st["lyapunov_fallback"]),
                       "rejudged": int(st["rejudged"]), "rejudge_err_q50": st["rejudge_err_q50"],
-                      "rejudge_err_q99": st["rejudge_err_q99"],
-                      "phases_ms": {k: st[k] for k in ("approach_ms", "mlp_ms", "fallback_ms", "overlay_ms")},
+                      "rejudge_err_q99": st["rejudge_err_q99"], "rejudge_err_max": st["rejudge_err_max"],
+                      "phases_ms": {k: st[k] for k in ("approach_ms", "coarse_ms", "mlp_ms", "fallback_ms",
+                                                       "overlay_ms")},
                       # algorithmic MLP FLOP per ray: 2 x (198*256 + 3*256*256 + 256*7) = 498,176
                       "mlp_tflops": (int(st["entered"]) * 498176 / (st["mlp_ms"] * 1e-3) / 1e12
                                      if st["mlp_ms"] > 0 else None)}
         view.close()
-        log(f"surrogate {field} {wl.width}x{wl.height}: {out[field]['speedup']}x, "
-            f"within 1e-3: {out[field]['frac_pixels_within_1e-3']:.4f}")
+        log(f"surrogate {field} {wl.width}x{wl.height}: {out[field]['speedup']}x, accepted "
+            f"{out[field]['accepted']}/{out[field]['entered']}, within tol {out[field]['accepted_within_tol']}")
     return out
 
 

@@ -146,7 +146,9 @@ typedef struct linon_device_buffers {
  * MLP (16 frequencies of the 6-float entry state = 198 features, 4 x 256 SiLU layers, 7 outputs:
  * exit point, exit direction, confidence) maps a ray's entry into the sphere of `radius` to its
  * exit, trained on uniform entry states labelled by the FP32 RK4 kernel (escape mode, no mesh,
- * step h).  Hidden layers run on BF16 MFMA with FP32 accumulation; the output layer is FP32. */
+ * step h, exits clipped to the sphere).  Hidden layers run on BF16 MFMA with FP32 accumulation;
+ * the output layer is FP32.  Errors are max over x, y, z of |exit - exit_RK4| / max(1, |exit_RK4|),
+ * as the parity tests measure them. */
 typedef struct linon_surrogate_config {
     int32_t field;             /* predefined field id */
     int32_t train_rays;        /* training-set size (hold-out: 1%, at most 65536) */
@@ -161,7 +163,11 @@ typedef struct linon_surrogate_config {
     float tolerance;           /* exit-point error the confidence head certifies */
     float rejudge_fraction;    /* frame: share of accepted rays re-run with RK4 to measure error */
     uint32_t seed;
-    int32_t pad[3];
+    float coarse_h;            /* > 0: the network learns the residual of the flow map at step h over
+                                  the flow map at this coarser step (a multi-fidelity surrogate: a frame
+                                  integrates each entered ray at coarse_h and adds the network's
+                                  correction); 0: the network predicts the exit itself */
+    int32_t pad[2];
 } linon_surrogate_config;
 
 typedef struct linon_surrogate_stats {
@@ -171,6 +177,7 @@ typedef struct linon_surrogate_stats {
     uint64_t frame_rays, entered, accepted, conf_fallback, lyapunov_fallback, rejudged;
     float rejudge_err_q50, rejudge_err_q99, rejudge_err_max, rejudge_frac_within_tol;
     float frame_ms, approach_ms, mlp_ms, fallback_ms, overlay_ms, rejudge_ms;
+    float coarse_ms;  /* the coarse flow map of the entered rays (frame_ms includes it) */
 } linon_surrogate_stats;
 
 typedef struct linon_context linon_context;
@@ -313,6 +320,21 @@ linon_status linon_surrogate_train(linon_context* ctx, const linon_surrogate_con
                                    linon_surrogate_stats* stats);
 linon_status linon_render_surrogate(linon_context* ctx, void* stream);
 linon_status linon_get_surrogate_stats(linon_context* ctx, linon_surrogate_stats* out);
+/* Per texel (W * H bytes, row 0 at the top): 1 where the last surrogate frame kept the
+ * surrogate's exit point, 0 where RK4 provided it or the ray never entered the sphere. */
+linon_status linon_read_surrogate_accepted(linon_context* ctx, uint8_t* mask, size_t bytes);
+/* Diagnostics: the trained network's raw outputs (exit point and direction in target units, the
+ * confidence logit) for n host entry states [n][6] into out [n][7], through the fused inference
+ * kernel a frame uses (fused = 1) or the training path's kernels (fused = 0). */
+linon_status linon_surrogate_infer(linon_context* ctx, const float* states, int n, float* out, int fused);
+/* Surrogate weights as a byte blob (no counterpart in the reference): the configuration, the
+ * Fourier frequencies, the FP32 master weights and biases of all five layers and the training
+ * statistics (not the optimiser state).  linon_surrogate_save writes it to `out` when `capacity`
+ * suffices and always stores its size in *needed (out = NULL only asks for the size);
+ * linon_surrogate_load checks the blob (magic, version, layer sizes, field) and installs it in
+ * place of any trained surrogate, ready for linon_render_surrogate. */
+linon_status linon_surrogate_save(linon_context* ctx, void* out, size_t capacity, size_t* needed);
+linon_status linon_surrogate_load(linon_context* ctx, const void* data, size_t bytes);
 
 /* Diagnostics (no counterpart in the reference): the flow map through the bounding sphere that
  * the surrogate learns, from host entry states.  Each state (p, q) of `states` [n][6] starts a ray

@@ -184,18 +184,20 @@ class SurrogateConfig(C.Structure):
     _fields_ = [("field", C.c_int32), ("train_rays", C.c_int32), ("batch", C.c_int32), ("steps", C.c_int32),
                 ("lr", C.c_float), ("h", C.c_float), ("max_steps", C.c_int32), ("radius", C.c_float),
                 ("conf_threshold", C.c_float), ("lyapunov_threshold", C.c_float), ("tolerance", C.c_float),
-                ("rejudge_fraction", C.c_float), ("seed", C.c_uint32), ("pad", C.c_int32 * 3)]
+                ("rejudge_fraction", C.c_float), ("seed", C.c_uint32), ("coarse_h", C.c_float),
+                ("pad", C.c_int32 * 2)]
 
     @classmethod
     def make(cls, field, train_rays=1 << 24, batch=65536, steps=2000, lr=1e-3, h=0.005, max_steps=4000,
              radius=2.0, conf_threshold=0.9, lyapunov_threshold=0.0955, tolerance=1e-3, rejudge_fraction=1 / 64,
-             seed=0):
+             seed=0, coarse_h=0.0):
         c = cls()
         c.field = FIELDS[field] if isinstance(field, str) else int(field)
         c.train_rays, c.batch, c.steps, c.lr, c.h = train_rays, batch, steps, lr, h
         c.max_steps, c.radius, c.conf_threshold = max_steps, radius, conf_threshold
         c.lyapunov_threshold, c.tolerance, c.rejudge_fraction, c.seed = lyapunov_threshold, tolerance, \
             rejudge_fraction, seed
+        c.coarse_h = coarse_h
         return c
 
 
@@ -207,7 +209,7 @@ class SurrogateStats(C.Structure):
                 ("rejudged", C.c_uint64), ("rejudge_err_q50", C.c_float), ("rejudge_err_q99", C.c_float),
                 ("rejudge_err_max", C.c_float), ("rejudge_frac_within_tol", C.c_float), ("frame_ms", C.c_float),
                 ("approach_ms", C.c_float), ("mlp_ms", C.c_float), ("fallback_ms", C.c_float),
-                ("overlay_ms", C.c_float), ("rejudge_ms", C.c_float)]
+                ("overlay_ms", C.c_float), ("rejudge_ms", C.c_float), ("coarse_ms", C.c_float)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -273,6 +275,10 @@ SIGNATURES = {
     "linon_selftest_gemm": (C.c_int, [C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P,
                                       C.c_int, _P, _P]),
     "linon_reset_timings": (C.c_int, [_P]),
+    "linon_surrogate_save": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "linon_surrogate_load": (C.c_int, [_P, _P, C.c_size_t]),
+    "linon_surrogate_infer": (C.c_int, [_P, _P, C.c_int, _P, C.c_int]),
+    "linon_read_surrogate_accepted": (C.c_int, [_P, _P, C.c_size_t]),
     "linon_flowmap": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(Integrator), C.c_float, _P, C.c_int, _P, _P]),
 }
 
@@ -545,6 +551,36 @@ class MainView:
         st = SurrogateStats()
         self._check(self._lib.linon_get_surrogate_stats(self._h, C.byref(st)))
         return st.as_dict()
+
+    def save_surrogate(self, path=None) -> bytes:
+        """The trained surrogate as a blob (linon_surrogate_save), also written to `path`."""
+        need = C.c_size_t(0)
+        self._check(self._lib.linon_surrogate_save(self._h, None, 0, C.byref(need)))
+        buf = np.zeros(need.value, np.uint8)
+        self._check(self._lib.linon_surrogate_save(self._h, _ptr(buf), buf.nbytes, C.byref(need)))
+        data = buf.tobytes()
+        if path is not None:
+            Path(path).write_bytes(data)
+        return data
+
+    def read_surrogate_accepted(self) -> np.ndarray:
+        """(H, W) bool: texels whose exit point the last surrogate frame took from the network."""
+        m = np.zeros((self.height, self.width), np.uint8)
+        self._check(self._lib.linon_read_surrogate_accepted(self._h, _ptr(m), m.nbytes))
+        return m.astype(bool)
+
+    def surrogate_infer(self, states, fused: bool = True) -> np.ndarray:
+        """Raw network outputs (n, 7) for (n, 6) entry states (linon_surrogate_infer)."""
+        x = np.ascontiguousarray(states, np.float32).reshape(-1, 6)
+        out = np.zeros((len(x), 7), np.float32)
+        self._check(self._lib.linon_surrogate_infer(self._h, _ptr(x), len(x), _ptr(out), 1 if fused else 0))
+        return out
+
+    def load_surrogate(self, src) -> None:
+        """Installs a saved surrogate (bytes, or a path to them) (linon_surrogate_load)."""
+        data = src if isinstance(src, (bytes, bytearray)) else Path(src).read_bytes()
+        buf = np.frombuffer(bytes(data), np.uint8).copy()
+        self._check(self._lib.linon_surrogate_load(self._h, _ptr(buf), buf.nbytes))
 
     def timings(self):
         """(raycast_ms, render_ms) arrays of the renders since reset_timings()."""

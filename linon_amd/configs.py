@@ -131,3 +131,35 @@ def batch_integrator(max_steps: int = 8000) -> Integrator:
     hit (no overlay in a batch), or after max_steps RK4 iterations."""
     return Integrator.make(h_initial=0.01, t_max=1e30, adaptive=False, max_steps=max_steps, mode=1,
                            escape_radius=2.0, samples_per_ray=0, sample_steps=0, need_mapping=False)
+
+
+# ---------------------------------------------------------------------------------------------
+# Learned flow-map surrogate (config 4): one per field at the workload's step h
+# ---------------------------------------------------------------------------------------------
+# coarse_h of the multi-fidelity form per field: the network learns the residual of the flow map
+# at h over the flow map at coarse_h.  Chosen from the coarse map's own error against h = 0.005
+# (max over x, y, z of |d| / max(1, |exit|), 2,000 uniform entry states, FP32 oracle): swirl at
+# 0.1 is within 1e-3 on all of them (q99 5e-4; at 0.2 on 71%, and the network's correction left
+# 6% of the accepted camera rays of the 1080p frame outside the tolerance), warp at 0.05 on 97%
+# (q50 1e-4), hot air at 0.02 on 89%; lens rays are mostly captured by its singularity at any
+# step, so it keeps the plain network (coarse_h 0), whose rays fall back to RK4.
+SURROGATE_COARSE_H = {"swirl": 0.1, "warp": 0.05, "hot_air": 0.02, "lens": 0.0}
+SURROGATE_STEPS = 4000
+SURROGATE_DIR = __import__("pathlib").Path(__file__).resolve().parent / "surrogates"
+
+
+def surrogate_config(field: str, wl: Workload = OVERLAY_1080P, steps: int = SURROGATE_STEPS, seed: int = 0):
+    from . import SurrogateConfig
+    return SurrogateConfig.make(field, train_rays=1 << 24, batch=65536, steps=steps, lr=1e-3, h=wl.h,
+                                max_steps=wl.max_steps, tolerance=1e-3, conf_threshold=0.9,
+                                coarse_h=SURROGATE_COARSE_H.get(field, 0.0), seed=seed)
+
+
+def surrogate_blob_name(field: str, wl: Workload = OVERLAY_1080P) -> str:
+    return f"{field}_h{wl.h:g}_max{wl.max_steps}_coarse{SURROGATE_COARSE_H.get(field, 0.0):g}.lnsg"
+
+
+def saved_surrogate(field: str, wl: Workload = OVERLAY_1080P):
+    """Path of the saved weights for this field and workload step, or None."""
+    p = SURROGATE_DIR / surrogate_blob_name(field, wl)
+    return p if p.exists() else None

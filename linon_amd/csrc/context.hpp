@@ -52,9 +52,13 @@ struct Surrogate {
     float* gW[5] = {};
     float* gb[5] = {};
     unsigned short* Wt[4] = {};  // BF16 copies [N][K] of the hidden layers
+    unsigned short* d_wpack = nullptr;  // the hidden layers in MFMA fragment order (fused inference)
     float* d_freqs = nullptr;
     // training
     float *d_states = nullptr, *d_targets = nullptr, *d_bstates = nullptr, *d_btargets = nullptr;
+    float* d_tscaled = nullptr;  // network targets (exit - coarse exit) * scale
+    float* d_bfine = nullptr;    // the mini-batch's RK4 exits
+    float scale_p = 0.5f, scale_d = 1.0f;  // output scales of the exit point and direction
     unsigned short* d_feat = nullptr;
     unsigned short* d_act[5] = {};
     float* d_z[4] = {};
@@ -75,6 +79,8 @@ struct Surrogate {
     unsigned short *f_feat = nullptr, *f_act0 = nullptr, *f_act1 = nullptr;
     float* f_out = nullptr;
     int* f_steps = nullptr;
+    float* f_base = nullptr;  // coarse flow map of the entered rays
+    unsigned char* f_mask = nullptr;  // per texel: the frame kept the surrogate's exit
     linon_surrogate_stats train_stats{};
     linon_surrogate_stats frame_stats{};
     void release();
@@ -161,9 +167,15 @@ struct Context {
     linon_status surrogate_forward(int n, const unsigned short* feat, unsigned short* const act[5],
                                    float* const z[4], hipStream_t s);
     linon_status surrogate_train(const linon_surrogate_config& cfg, linon_surrogate_stats* stats);
-    linon_status surrogate_eval(const float* states, const float* targets, int n, std::vector<float>& err);
+    linon_status surrogate_eval(const float* states, const float* targets, const float* fine, int n,
+                                std::vector<float>& err);
     linon_status surrogate_frame(hipStream_t s);
+    linon_status surrogate_save(void* out, size_t capacity, size_t* needed);
+    linon_status surrogate_pack();  // refreshes d_wpack from the master weights
+    linon_status surrogate_infer(const float* states, int n, float* out, bool fused);
+    linon_status surrogate_load(const void* data, size_t bytes);
     StepParams surrogate_step_params() const;
+    StepParams surrogate_coarse_step_params() const;
 
     void fill_scene(SceneParams& S) const;
     void fill_raycast_params(RaycastParams& P, const linon_integrator& ig) const;

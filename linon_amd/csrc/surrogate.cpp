@@ -1,4 +1,4 @@
-// Learned flow-map surrogate: host side (training loop, frame pipeline, statistics, save/load).
+// Learned flow-map surrogate: host side (training loop, frame pipeline, statistics).
 //
 // Training follows the usual data-parallel recipe on one device: uniform entry states ->
 // labels from the FP32 RK4 flow-map kernel -> mini-batches gathered by a counter-based hash ->
@@ -63,10 +63,10 @@ void Surrogate::release() {
         if (Wt[l]) (void)hipFree(Wt[l]);
         Wt[l] = nullptr;
     }
-    void* bufs[] = {d_freqs, d_states, d_targets, d_bstates, d_btargets, d_feat, d_act[0], d_act[1], d_act[2],
-                    d_act[3], d_act[4], d_z[0], d_z[1], d_z[2], d_z[3], d_out, d_dout, d_err, d_dz[0], d_dz[1],
-                    d_part, d_loss, f_entry, f_entered, f_states, f_texel, f_count, f_pred, f_accepted, f_list,
-                    f_exits, f_errs, f_feat, f_act0, f_act1, f_out, f_steps};
+    void* bufs[] = {d_wpack, d_freqs, d_states, d_targets, d_bstates, d_btargets, d_tscaled, d_bfine, d_feat, d_act[0],
+                    d_act[1], d_act[2], d_act[3], d_act[4], d_z[0], d_z[1], d_z[2], d_z[3], d_out, d_dout, d_err,
+                    d_dz[0], d_dz[1], d_part, d_loss, f_entry, f_entered, f_states, f_texel, f_count, f_pred,
+                    f_accepted, f_list, f_exits, f_errs, f_feat, f_act0, f_act1, f_out, f_steps, f_base, f_mask};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     *this = Surrogate();
@@ -167,13 +167,81 @@ linon_status Context::surrogate_forward(int n, const lm::u16* feat, lm::u16* con
     return LINON_OK;
 }
 
+linon_status Context::surrogate_pack() {
+    Surrogate& S = sg;
+    if (!S.d_wpack) LS_CHECK(this, hipMalloc(&S.d_wpack, (size_t)LM_PACK_TOTAL * sizeof(lm::u16)));
+    for (int l = 0; l < 4; ++l) LS_CHECK(this, sg_mlp_pack(S.W[l], kLayerK[l], S.d_wpack + lm::pack_offset(l), stream));
+    LS_CHECK(this, hipStreamSynchronize(stream));
+    return LINON_OK;
+}
+
+// Raw network outputs for n states, through the fused inference kernel or the training path's
+// kernels (features, four GEMMs, output layer); both give the same bits.
+linon_status Context::surrogate_infer(const float* states, int n, float* out, bool fused) {
+    Surrogate& S = sg;
+    if (!S.trained) return fail(LINON_ERR_STATE, "surrogate_infer: no trained surrogate");
+    if (!states || !out || n < 1 || n > (1 << 24)) return fail(LINON_ERR_ARGUMENT, "surrogate_infer: bad arguments");
+    LS_CHECK(this, hipSetDevice(device));
+    hipStream_t s = stream;
+    float *d_in = nullptr, *d_out = nullptr;
+    lm::u16 *feat = nullptr, *a0 = nullptr, *a1 = nullptr;
+    auto cleanup = [&]() {
+        for (void* p : {(void*)d_in, (void*)d_out, (void*)feat, (void*)a0, (void*)a1})
+            if (p) (void)hipFree(p);
+    };
+    linon_status st = LINON_OK;
+    if (dalloc(d_in, (size_t)n * 6) != hipSuccess || dalloc(d_out, (size_t)n * LS_OUT) != hipSuccess ||
+        (!fused && (dalloc(feat, (size_t)n * LS_FEATS_PAD) != hipSuccess ||
+                    dalloc(a0, (size_t)n * LS_HIDDEN) != hipSuccess || dalloc(a1, (size_t)n * LS_HIDDEN) != hipSuccess))) {
+        cleanup();
+        return fail(LINON_ERR_HIP, "surrogate_infer: out of device memory");
+    }
+    do {
+        if (hipMemcpy(d_in, states, (size_t)n * 24, hipMemcpyHostToDevice) != hipSuccess) {
+            st = fail(LINON_ERR_HIP, "surrogate_infer: copy");
+            break;
+        }
+        if (fused) {
+            lm::MlpInferParams M;
+            std::memset(&M, 0, sizeof(M));
+            M.states = d_in;
+            M.n = n;
+            M.radius = S.cfg.radius;
+            M.freqs = S.d_freqs;
+            M.wpack = S.d_wpack;
+            for (int l = 0; l < 4; ++l) M.bias[l] = S.b[l];
+            M.w_out = S.W[4];
+            M.b_out = S.b[4];
+            M.out = d_out;
+            if (sg_mlp_infer(M, s) != hipSuccess) { st = fail(LINON_ERR_HIP, "surrogate_infer: fused"); break; }
+        } else {
+            ls::SurrogateIO io{d_in, n, S.cfg.radius, S.d_freqs, feat};
+            if (sg_features(io, s) != hipSuccess) { st = fail(LINON_ERR_HIP, "surrogate_infer: features"); break; }
+            lm::u16* const acts[5] = {feat, a0, a1, a0, a1};
+            st = surrogate_forward(n, feat, acts, nullptr, s);
+            if (st != LINON_OK) break;
+            if (sg_out_layer(a1, S.W[4], S.b[4], n, d_out, s) != hipSuccess) {
+                st = fail(LINON_ERR_HIP, "surrogate_infer: output layer");
+                break;
+            }
+        }
+        if (hipStreamSynchronize(s) != hipSuccess ||
+            hipMemcpy(out, d_out, (size_t)n * LS_OUT * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            st = fail(LINON_ERR_HIP, "surrogate_infer: copy back");
+    } while (false);
+    (void)hipDeviceSynchronize();
+    cleanup();
+    return st;
+}
+
 linon_status Context::surrogate_train(const linon_surrogate_config& cfg, linon_surrogate_stats* stats) {
     if (!sg_field_supported(cfg.field))
         return fail(LINON_ERR_ARGUMENT, "surrogate: field must be swirl, lens, hot-air, warp, identity or "
                                         "mirage (spherical sigmoid)");
     if (cfg.train_rays < 1024 || cfg.batch < 256 || cfg.batch > (1 << 20) || cfg.batch % 64 || cfg.steps < 0 ||
-        !(cfg.lr > 0.0f) || !(cfg.h > 0.0f) || cfg.max_steps < 1 || !(cfg.radius > 0.0f))
-        return fail(LINON_ERR_ARGUMENT, "surrogate: bad configuration");
+        !(cfg.lr > 0.0f) || !(cfg.h > 0.0f) || cfg.max_steps < 1 || !(cfg.radius > 0.0f) ||
+        !(cfg.coarse_h >= 0.0f) || (cfg.coarse_h > 0.0f && !(cfg.coarse_h > cfg.h)))
+        return fail(LINON_ERR_ARGUMENT, "surrogate: bad configuration (coarse_h must be 0 or above h)");
     Surrogate& S = sg;
     S.release();
     S.cfg = cfg;
@@ -200,12 +268,41 @@ linon_status Context::surrogate_train(const linon_surrogate_config& cfg, linon_s
     F.exits = S.d_targets;
     LS_CHECK(this, wgsl_math_for(cfg.field) ? linon_wgslmath::sg_flowmap(cfg.field, F, s)
                                               : sg_flowmap(cfg.field, F, s));
+    // the coarse flow map of the same states, and the network targets: (exit - coarse) * scale,
+    // each scale making its residual's RMS 1/4; without a coarse map, exit / R and the direction
+    float* base = nullptr;
+    if (cfg.coarse_h > 0.0f) {
+        LS_CHECK(this, dalloc(base, (size_t)N * 6));
+        ls::FlowmapParams Fc = F;
+        Fc.step = surrogate_coarse_step_params();
+        Fc.exits = base;
+        LS_CHECK(this, wgsl_math_for(cfg.field) ? linon_wgslmath::sg_flowmap(cfg.field, Fc, s)
+                                                  : sg_flowmap(cfg.field, Fc, s));
+        float* acc = nullptr;
+        LS_CHECK(this, dalloc(acc, 2));
+        LS_CHECK(this, hipMemsetAsync(acc, 0, 8, s));
+        LS_CHECK(this, sg_residual_sumsq(S.d_targets, base, N, acc, s));
+        float h_acc[2] = {0.0f, 0.0f};
+        LS_CHECK(this, hipMemcpyAsync(h_acc, acc, 8, hipMemcpyDeviceToHost, s));
+        LS_CHECK(this, hipStreamSynchronize(s));
+        (void)hipFree(acc);
+        const double rms_p = std::sqrt(h_acc[0] / (3.0 * N)), rms_d = std::sqrt(h_acc[1] / (3.0 * N));
+        S.scale_p = (float)(0.25 / std::max(rms_p, 1e-7));
+        S.scale_d = (float)(0.25 / std::max(rms_d, 1e-7));
+    } else {
+        S.scale_p = 1.0f / cfg.radius;
+        S.scale_d = 1.0f;
+    }
+    LS_CHECK(this, dalloc(S.d_tscaled, (size_t)N * 6));
+    LS_CHECK(this, sg_make_targets(S.d_targets, base, N, S.scale_p, S.scale_d, S.d_tscaled, s));
     LS_CHECK(this, hipStreamSynchronize(s));
+    if (base) (void)hipFree(base);
     auto t1 = std::chrono::steady_clock::now();
 
     // ---- buffers ----
     LS_CHECK(this, dalloc(S.d_bstates, (size_t)B * 6));
     LS_CHECK(this, dalloc(S.d_btargets, (size_t)B * 6));
+    LS_CHECK(this, dalloc(S.d_bfine, (size_t)B * 6));
     LS_CHECK(this, dalloc(S.d_feat, (size_t)B * LS_FEATS_PAD));
     for (int l = 1; l < 5; ++l) LS_CHECK(this, dalloc(S.d_act[l], (size_t)B * LS_HIDDEN));
     for (int l = 0; l < 4; ++l) LS_CHECK(this, dalloc(S.d_z[l], (size_t)B * LS_HIDDEN));
@@ -226,8 +323,8 @@ linon_status Context::surrogate_train(const linon_surrogate_config& cfg, linon_s
     const float conf_w = 0.1f;
     float last_loss = 0.0f;
     for (int step = 0; step < cfg.steps; ++step) {
-        LS_CHECK(this, sg_gather(S.d_states, S.d_targets, n_train, cfg.seed ^ 0x5DEECE66Dull, step, B, cfg.radius,
-                                 S.d_bstates, S.d_btargets, s));
+        LS_CHECK(this, sg_gather(S.d_states, S.d_tscaled, S.d_targets, n_train, cfg.seed ^ 0x5DEECE66Dull, step, B,
+                                 S.d_bstates, S.d_btargets, S.d_bfine, s));
         ls::SurrogateIO io{S.d_bstates, B, cfg.radius, S.d_freqs, S.d_feat};
         LS_CHECK(this, sg_features(io, s));
         lm::u16* const acts[5] = {S.d_feat, S.d_act[1], S.d_act[2], S.d_act[3], S.d_act[4]};
@@ -237,8 +334,8 @@ linon_status Context::surrogate_train(const linon_surrogate_config& cfg, linon_s
         LS_CHECK(this, sg_out_layer(S.d_act[4], S.W[4], S.b[4], B, S.d_out, s));
         const bool want_loss = (step % 50 == 0) || step == cfg.steps - 1;
         if (want_loss) LS_CHECK(this, hipMemsetAsync(S.d_loss, 0, 4, s));
-        LS_CHECK(this, sg_loss(S.d_out, S.d_btargets, B, cfg.radius, cfg.tolerance, conf_w, S.d_dout, S.d_err,
-                               S.d_loss, s));
+        LS_CHECK(this, sg_loss(S.d_out, S.d_btargets, S.d_bfine, B, S.scale_p, cfg.tolerance, conf_w, S.d_dout,
+                               S.d_err, S.d_loss, s));
         // output layer (FP32)
         LS_CHECK(this, sg_out_layer_grad(S.d_act[4], S.d_dout, B, S.gW[4], S.gb[4], s));
         LS_CHECK(this, sg_out_layer_back(S.d_dout, S.W[4], S.d_z[3], B, S.d_dz[0], s));
@@ -287,16 +384,17 @@ linon_status Context::surrogate_train(const linon_surrogate_config& cfg, linon_s
                 cur ^= 1;
             }
         }
-        // Adam
+        // Adam, the learning rate on a cosine from lr down to lr / 100 over the run
         ++S.adam_t;
         const float bc1 = 1.0f - std::pow(b1, (float)S.adam_t), bc2 = 1.0f - std::pow(b2, (float)S.adam_t);
+        const float lr = cfg.lr * (0.01f + 0.99f * 0.5f * (1.0f + std::cos(3.14159265f * step / std::max(1, cfg.steps))));
         for (int l = 0; l < 5; ++l) {
             const int K = l < 4 ? kLayerK[l] : LS_HIDDEN;
             const int Nn = l < 4 ? LS_HIDDEN : LS_OUT;
-            LS_CHECK(this, sg_adam(S.W[l], S.mW[l], S.vW[l], S.gW[l], K, Nn, cfg.lr, b1, b2, eps, bc1, bc2,
+            LS_CHECK(this, sg_adam(S.W[l], S.mW[l], S.vW[l], S.gW[l], K, Nn, lr, b1, b2, eps, bc1, bc2,
                                    l < 4 ? S.Wt[l] : nullptr, K, s));
-            LS_CHECK(this, sg_adam(S.b[l], S.mb[l], S.vb[l], S.gb[l], 1, Nn, cfg.lr, b1, b2, eps, bc1, bc2, nullptr,
-                                   0, s));
+            LS_CHECK(this, sg_adam(S.b[l], S.mb[l], S.vb[l], S.gb[l], 1, Nn, lr, b1, b2, eps, bc1, bc2, nullptr, 0,
+                                   s));
         }
         if (want_loss) {
             LS_CHECK(this, hipMemcpyAsync(&last_loss, S.d_loss, 4, hipMemcpyDeviceToHost, s));
@@ -309,7 +407,10 @@ linon_status Context::surrogate_train(const linon_surrogate_config& cfg, linon_s
 
     // ---- hold-out error (the last n_hold states, never sampled for training) ----
     std::vector<float> err;
-    st = surrogate_eval(S.d_states + 6 * (size_t)n_train, S.d_targets + 6 * (size_t)n_train, n_hold, err);
+    st = surrogate_eval(S.d_states + 6 * (size_t)n_train, S.d_tscaled + 6 * (size_t)n_train,
+                        S.d_targets + 6 * (size_t)n_train, n_hold, err);
+    if (st != LINON_OK) return st;
+    st = surrogate_pack();
     if (st != LINON_OK) return st;
     S.trained = true;
     if (stats) {
@@ -344,22 +445,32 @@ StepParams Context::surrogate_step_params() const {
     return I;
 }
 
-// Exit-point error of the surrogate on n states against their labels (host copy of errors).
-linon_status Context::surrogate_eval(const float* states, const float* targets, int n, std::vector<float>& err) {
+// The coarse flow map: the same loop at step coarse_h, with the step cap scaled to the same
+// path length.
+StepParams Context::surrogate_coarse_step_params() const {
+    StepParams I = surrogate_step_params();
+    I.h_initial = sg.cfg.coarse_h;
+    I.max_steps = std::max(1, (int)std::ceil((double)sg.cfg.max_steps * sg.cfg.h / sg.cfg.coarse_h));
+    return I;
+}
+
+// Exit-point error of the surrogate on n states (host copy of the errors): `targets` are the
+// network targets of the states, `fine` their RK4 exits.
+linon_status Context::surrogate_eval(const float* states, const float* targets, const float* fine, int n,
+                                     std::vector<float>& err) {
     Surrogate& S = sg;
     hipStream_t s = stream;
     lm::u16* feat = nullptr;
     lm::u16 *a0 = nullptr, *a1 = nullptr;
-    float *out = nullptr, *dout = nullptr, *e = nullptr, *loss = nullptr, *scaled = nullptr;
+    float *out = nullptr, *dout = nullptr, *e = nullptr, *loss = nullptr;
     auto cleanup = [&]() {
-        for (void* p : {(void*)feat, (void*)a0, (void*)a1, (void*)out, (void*)dout, (void*)e, (void*)loss,
-                        (void*)scaled})
+        for (void* p : {(void*)feat, (void*)a0, (void*)a1, (void*)out, (void*)dout, (void*)e, (void*)loss})
             if (p) (void)hipFree(p);
     };
     if (dalloc(feat, (size_t)n * LS_FEATS_PAD) != hipSuccess || dalloc(a0, (size_t)n * LS_HIDDEN) != hipSuccess ||
         dalloc(a1, (size_t)n * LS_HIDDEN) != hipSuccess || dalloc(out, (size_t)n * LS_OUT) != hipSuccess ||
         dalloc(dout, (size_t)n * LS_OUT) != hipSuccess || dalloc(e, (size_t)n) != hipSuccess ||
-        dalloc(loss, 1) != hipSuccess || dalloc(scaled, (size_t)n * 6) != hipSuccess) {
+        dalloc(loss, 1) != hipSuccess) {
         cleanup();
         return fail(LINON_ERR_HIP, "surrogate_eval: out of device memory");
     }
@@ -371,20 +482,8 @@ linon_status Context::surrogate_eval(const float* states, const float* targets, 
         st = surrogate_forward(n, feat, acts, nullptr, s);
         if (st != LINON_OK) break;
         if (sg_out_layer(a1, S.W[4], S.b[4], n, out, s) != hipSuccess) { st = fail(LINON_ERR_HIP, "eval out"); break; }
-        // labels scaled like the training targets
-        std::vector<float> t((size_t)n * 6);
-        if (hipMemcpy(t.data(), targets, t.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) {
-            st = fail(LINON_ERR_HIP, "eval copy");
-            break;
-        }
-        for (size_t i = 0; i < (size_t)n; ++i)
-            for (int k = 0; k < 3; ++k) t[6 * i + k] /= S.cfg.radius;
-        if (hipMemcpy(scaled, t.data(), t.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemset(loss, 0, 4) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
-            st = fail(LINON_ERR_HIP, "eval copy");
-            break;
-        }
-        if (sg_loss(out, scaled, n, S.cfg.radius, S.cfg.tolerance, 0.1f, dout, e, loss, s) != hipSuccess) {
+        if (hipMemsetAsync(loss, 0, 4, s) != hipSuccess ||
+            sg_loss(out, targets, fine, n, S.scale_p, S.cfg.tolerance, 0.1f, dout, e, loss, s) != hipSuccess) {
             st = fail(LINON_ERR_HIP, "eval loss");
             break;
         }
@@ -424,13 +523,12 @@ linon_status Context::surrogate_frame(hipStream_t s) {
         LS_CHECK(this, dalloc(S.f_list, (size_t)npx));
         LS_CHECK(this, dalloc(S.f_exits, (size_t)npx * 6));
         LS_CHECK(this, dalloc(S.f_errs, (size_t)npx));
-        LS_CHECK(this, dalloc(S.f_feat, (size_t)npx * LS_FEATS_PAD));
-        LS_CHECK(this, dalloc(S.f_act0, (size_t)npx * LS_HIDDEN));
-        LS_CHECK(this, dalloc(S.f_act1, (size_t)npx * LS_HIDDEN));
         LS_CHECK(this, dalloc(S.f_out, (size_t)npx * LS_OUT));
+        LS_CHECK(this, dalloc(S.f_base, (size_t)npx * 6));
+        LS_CHECK(this, dalloc(S.f_mask, (size_t)npx));
         S.frame_px = npx;
     }
-    hipEvent_t e[6];
+    hipEvent_t e[7];  // e[6]: after the coarse flow map, inside the MLP phase
     for (auto& x : e) LS_CHECK(this, hipEventCreate(&x));
     auto destroy = [&]() {
         for (auto& x : e) (void)hipEventDestroy(x);
@@ -459,21 +557,37 @@ linon_status Context::surrogate_frame(hipStream_t s) {
     LS_CHECK(this, hipMemcpyAsync(&n_in, S.f_count, 4, hipMemcpyDeviceToHost, s));
     LS_CHECK(this, hipStreamSynchronize(s));
     LS_CHECK(this, hipEventRecord(e[1], s));
-    // 2. MLP
+    // 2. MLP (over the coarse flow map of the entered rays, when the network learnt a residual)
     int n_fb = 0;
+    const bool coarse = S.cfg.coarse_h > 0.0f;
     if (n_in > 0) {
-        ls::SurrogateIO io{S.f_states, n_in, S.cfg.radius, S.d_freqs, S.f_feat};
-        LS_CHECK(this, sg_features(io, s));
-        lm::u16* const acts[5] = {S.f_feat, S.f_act0, S.f_act1, S.f_act0, S.f_act1};
-        linon_status st = surrogate_forward(n_in, S.f_feat, acts, nullptr, s);
-        if (st != LINON_OK) {
-            destroy();
-            return st;
+        if (coarse) {
+            ls::FlowmapParams Fc;
+            std::memset(&Fc, 0, sizeof(Fc));
+            Fc.step = surrogate_coarse_step_params();
+            Fc.states = S.f_states;
+            Fc.n = n_in;
+            Fc.exits = S.f_base;
+            LS_CHECK(this, wgsl_math_for(S.cfg.field) ? linon_wgslmath::sg_flowmap(S.cfg.field, Fc, s)
+                                                        : sg_flowmap(S.cfg.field, Fc, s));
         }
-        LS_CHECK(this, sg_out_layer(S.f_act1, S.W[4], S.b[4], n_in, S.f_out, s));
+        LS_CHECK(this, hipEventRecord(e[6], s));
+        // features, the four hidden layers and the output layer in one fused launch
+        lm::MlpInferParams M;
+        std::memset(&M, 0, sizeof(M));
+        M.states = S.f_states;
+        M.n = n_in;
+        M.radius = S.cfg.radius;
+        M.freqs = S.d_freqs;
+        M.wpack = S.d_wpack;
+        for (int l = 0; l < 4; ++l) M.bias[l] = S.b[l];
+        M.w_out = S.W[4];
+        M.b_out = S.b[4];
+        M.out = S.f_out;
+        LS_CHECK(this, sg_mlp_infer(M, s));
         LS_CHECK(this, sg_fill_i32(S.f_count, 1, 0, s));
-        LS_CHECK(this, sg_accept(S.f_out, S.f_texel, n_in, S.cfg.radius, S.cfg.conf_threshold, d_mapping, S.f_pred,
-                                 S.f_list, S.f_count, S.f_accepted, s));
+        LS_CHECK(this, sg_accept(S.f_out, coarse ? S.f_base : nullptr, S.f_texel, n_in, S.scale_p,
+                                 S.cfg.conf_threshold, d_mapping, S.f_pred, S.f_list, S.f_count, S.f_accepted, s));
         LS_CHECK(this, hipMemcpyAsync(&n_fb, S.f_count, 4, hipMemcpyDeviceToHost, s));
         LS_CHECK(this, hipStreamSynchronize(s));
     }
@@ -525,6 +639,7 @@ linon_status Context::surrogate_frame(hipStream_t s) {
         }
     }
     LS_CHECK(this, hipEventRecord(e[4], s));
+    LS_CHECK(this, sg_accepted_mask(S.f_accepted, S.f_texel, n_in, npx, S.f_mask, s));
     // 5. re-judge a sample of the accepted rays
     int n_rj = 0;
     std::vector<float> rj;
@@ -557,7 +672,12 @@ linon_status Context::surrogate_frame(hipStream_t s) {
     float ms[5];
     for (int k = 0; k < 5; ++k) (void)hipEventElapsedTime(&ms[k], e[k], e[k + 1]);
     F2.approach_ms = ms[0];
+    F2.coarse_ms = 0.0f;
     F2.mlp_ms = ms[1];
+    if (n_in > 0) {  // the MLP phase: the coarse flow map, then the network
+        (void)hipEventElapsedTime(&F2.coarse_ms, e[1], e[6]);
+        (void)hipEventElapsedTime(&F2.mlp_ms, e[6], e[2]);
+    }
     F2.fallback_ms = ms[2];
     F2.overlay_ms = ms[3];
     F2.rejudge_ms = ms[4];
@@ -572,6 +692,117 @@ linon_status Context::surrogate_frame(hipStream_t s) {
     }
     ++S.frames;
     destroy();
+    return LINON_OK;
+}
+
+namespace {
+
+// Saved-surrogate blob: header, then per layer l = 0..4 the [K][N] FP32 weights and N biases.
+struct SurrogateBlobHeader {
+    char magic[8];
+    uint32_t version, header_bytes;
+    uint32_t dims[5][2];  // K, N per layer
+    float freqs[LS_FREQS];
+    float scale[2];  // output scales of the exit point and direction
+    linon_surrogate_config cfg;
+    linon_surrogate_stats stats;
+};
+const char kBlobMagic[8] = {'L', 'I', 'N', 'O', 'N', 'S', 'G', '1'};
+
+size_t blob_bytes() {
+    size_t n = sizeof(SurrogateBlobHeader);
+    for (int l = 0; l < 5; ++l) {
+        const size_t K = l < 4 ? kLayerK[l] : LS_HIDDEN, N = l < 4 ? LS_HIDDEN : LS_OUT;
+        n += (K * N + N) * sizeof(float);
+    }
+    return n;
+}
+
+}  // namespace
+
+linon_status Context::surrogate_save(void* out, size_t capacity, size_t* needed) {
+    const size_t total = blob_bytes();
+    if (needed) *needed = total;
+    if (!out) return LINON_OK;
+    if (!sg.trained) return fail(LINON_ERR_STATE, "surrogate_save: no trained surrogate");
+    if (capacity < total) return fail(LINON_ERR_ARGUMENT, "surrogate_save: buffer smaller than the blob");
+    LS_CHECK(this, hipSetDevice(device));
+    LS_CHECK(this, hipDeviceSynchronize());
+    SurrogateBlobHeader h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, kBlobMagic, 8);
+    h.version = 1;
+    h.header_bytes = sizeof(h);
+    for (int l = 0; l < 5; ++l) {
+        h.dims[l][0] = l < 4 ? kLayerK[l] : LS_HIDDEN;
+        h.dims[l][1] = l < 4 ? LS_HIDDEN : LS_OUT;
+    }
+    LS_CHECK(this, hipMemcpy(h.freqs, sg.d_freqs, sizeof(h.freqs), hipMemcpyDeviceToHost));
+    h.scale[0] = sg.scale_p;
+    h.scale[1] = sg.scale_d;
+    h.cfg = sg.cfg;
+    h.stats = sg.train_stats;
+    char* p = static_cast<char*>(out);
+    std::memcpy(p, &h, sizeof(h));
+    p += sizeof(h);
+    for (int l = 0; l < 5; ++l) {
+        const size_t K = h.dims[l][0], N = h.dims[l][1];
+        LS_CHECK(this, hipMemcpy(p, sg.W[l], K * N * 4, hipMemcpyDeviceToHost));
+        p += K * N * 4;
+        LS_CHECK(this, hipMemcpy(p, sg.b[l], N * 4, hipMemcpyDeviceToHost));
+        p += N * 4;
+    }
+    return LINON_OK;
+}
+
+linon_status Context::surrogate_load(const void* data, size_t bytes) {
+    const size_t total = blob_bytes();
+    SurrogateBlobHeader h;
+    if (!data || bytes != total) return fail(LINON_ERR_ARGUMENT, "surrogate_load: blob size mismatch");
+    std::memcpy(&h, data, sizeof(h));
+    if (std::memcmp(h.magic, kBlobMagic, 8) != 0 || h.version != 1 || h.header_bytes != sizeof(h))
+        return fail(LINON_ERR_ARGUMENT, "surrogate_load: not a surrogate blob of this version");
+    for (int l = 0; l < 5; ++l)
+        if (h.dims[l][0] != (uint32_t)(l < 4 ? kLayerK[l] : LS_HIDDEN) || h.dims[l][1] != (uint32_t)(l < 4 ? LS_HIDDEN : LS_OUT))
+            return fail(LINON_ERR_ARGUMENT, "surrogate_load: layer sizes differ from this build's network");
+    if (!sg_field_supported(h.cfg.field) || !(h.cfg.radius > 0.0f) || !(h.cfg.h > 0.0f) || h.cfg.max_steps < 1 ||
+        !(h.cfg.coarse_h >= 0.0f) || !(h.scale[0] > 0.0f) || !(h.scale[1] > 0.0f))
+        return fail(LINON_ERR_ARGUMENT, "surrogate_load: configuration out of range");
+    LS_CHECK(this, hipSetDevice(device));
+    LS_CHECK(this, hipDeviceSynchronize());
+    Surrogate& S = sg;
+    S.release();
+    S.cfg = h.cfg;
+    linon_status st = surrogate_alloc_params();
+    if (st != LINON_OK) return st;
+    LS_CHECK(this, hipMemcpy(S.d_freqs, h.freqs, sizeof(h.freqs), hipMemcpyHostToDevice));
+    const char* p = static_cast<const char*>(data) + sizeof(h);
+    for (int l = 0; l < 5; ++l) {
+        const size_t K = h.dims[l][0], N = h.dims[l][1];
+        std::vector<float> w(K * N);
+        std::memcpy(w.data(), p, K * N * 4);
+        p += K * N * 4;
+        LS_CHECK(this, hipMemcpy(S.W[l], w.data(), K * N * 4, hipMemcpyHostToDevice));
+        LS_CHECK(this, hipMemcpy(S.b[l], p, N * 4, hipMemcpyHostToDevice));
+        p += N * 4;
+        if (l < 4) {  // the BF16 [N][K] copy the GEMMs read
+            std::vector<uint16_t> wt(K * N);
+            for (size_t k = 0; k < K; ++k)
+                for (size_t n = 0; n < N; ++n) {
+                    const float r = host_bf16_round(w[k * N + n]);
+                    uint32_t u;
+                    std::memcpy(&u, &r, 4);
+                    wt[n * K + k] = (uint16_t)(u >> 16);
+                }
+            LS_CHECK(this, hipMemcpy(S.Wt[l], wt.data(), wt.size() * 2, hipMemcpyHostToDevice));
+        }
+    }
+    S.scale_p = h.scale[0];
+    S.scale_d = h.scale[1];
+    S.train_stats = h.stats;
+    st = surrogate_pack();
+    if (st != LINON_OK) return st;
+    S.trained = true;
     return LINON_OK;
 }
 
@@ -595,6 +826,38 @@ linon_status linon_render_surrogate(linon_context* ctx, void* stream) {
     if (!c) return LINON_ERR_ARGUMENT;
     c->err.clear();
     return c->surrogate_frame(stream ? reinterpret_cast<hipStream_t>(stream) : c->stream);
+}
+
+linon_status linon_surrogate_save(linon_context* ctx, void* out, size_t capacity, size_t* needed) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return LINON_ERR_ARGUMENT;
+    c->err.clear();
+    return c->surrogate_save(out, capacity, needed);
+}
+
+linon_status linon_surrogate_load(linon_context* ctx, const void* data, size_t bytes) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return LINON_ERR_ARGUMENT;
+    c->err.clear();
+    return c->surrogate_load(data, bytes);
+}
+
+linon_status linon_read_surrogate_accepted(linon_context* ctx, uint8_t* mask, size_t bytes) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return LINON_ERR_ARGUMENT;
+    c->err.clear();
+    if (!mask) return c->fail(LINON_ERR_ARGUMENT, "read_surrogate_accepted: null buffer");
+    if (!c->sg.frames || !c->sg.f_mask || c->sg.frame_px != (int)(c->width * c->height))
+        return c->fail(LINON_ERR_STATE, "read_surrogate_accepted: no surrogate frame at the current size");
+    return c->read(c->sg.f_mask, mask, (size_t)c->sg.frame_px, bytes, "read_surrogate_accepted");
+}
+
+linon_status linon_surrogate_infer(linon_context* ctx, const float* states, int n, float* out, int fused) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return LINON_ERR_ARGUMENT;
+    c->err.clear();
+    if (fused != 0 && fused != 1) return c->fail(LINON_ERR_ARGUMENT, "surrogate_infer: fused must be 0 or 1");
+    return c->surrogate_infer(states, n, out, fused == 1);
 }
 
 linon_status linon_get_surrogate_stats(linon_context* ctx, linon_surrogate_stats* out) {

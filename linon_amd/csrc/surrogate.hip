@@ -6,6 +6,7 @@
 #include "device/fields.hpp"
 #include "device/gemm_nt.hpp"
 #include "device/mlp.hpp"
+#include "device/mlp_fused.hpp"
 #include "device/surrogate.hpp"
 #include "surrogate_kernels.hpp"
 
@@ -17,6 +18,9 @@ __global__ void __launch_bounds__(256) gemm_kernel(lm::GemmParams P) { lm::gemm_
 __global__ void __launch_bounds__(256) gemm_nt_kernel(lm::GemmParams P) { lm::gemm_nt_body(P); }
 
 __global__ void __launch_bounds__(256) features_kernel(ls::SurrogateIO S) { ls::features_body(S); }
+__global__ void __launch_bounds__(256) mlp_pack_kernel(const float* W, int K, u16* dst) { lm::mlp_pack_body(W, K, dst); }
+// two 128-ray workgroups per CU (75 KB of LDS each)
+__global__ void __launch_bounds__(256, 2) mlp_infer_kernel(lm::MlpInferParams P) { lm::mlp_infer_body(P); }
 
 template <class F>
 __global__ void __launch_bounds__(256) approach_kernel(ls::ApproachParams A) {
@@ -69,11 +73,52 @@ __global__ void __launch_bounds__(256) pick_rejudge_kernel(const int* accepted, 
     list[slot] = i;
 }
 
-// Gathers a mini-batch: rows idx[i] of the training states/targets (idx from a counter-based
-// hash of (seed, step, i)), scaled targets (p / R, d).
-__global__ void __launch_bounds__(256) gather_batch_kernel(const float* states, const float* targets, int n_data,
-                                                           unsigned long long seed, int step, int batch, float radius,
-                                                           float* bstates, float* btargets) {
+// Network targets of the training set: (exit - base) * scale, per row (point scale for the
+// three exit-point values, direction scale for the exit direction); base = the coarse flow map,
+// or none (0) when the network predicts the exit itself.
+__global__ void __launch_bounds__(256) make_targets_kernel(const float* fine, const float* base, int n,
+                                                           float scale_p, float scale_d, float* tgt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 6 * n) return;
+    const int k = t % 6;
+    const float b = base ? base[t] : 0.0f;
+    tgt[t] = (fine[t] - b) * (k < 3 ? scale_p : scale_d);
+}
+
+// Sums of squares of (fine - base) over all rows: acc[0] over the exit-point values, acc[1]
+// over the directions (block reduction, one atomic per block and accumulator).
+__global__ void __launch_bounds__(256) residual_sumsq_kernel(const float* fine, const float* base, int n, float* acc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    float sp = 0.0f, sd = 0.0f;
+    if (t < n) {
+        for (int k = 0; k < 6; ++k) {
+            const float d = fine[6 * (size_t)t + k] - (base ? base[6 * (size_t)t + k] : 0.0f);
+            (k < 3 ? sp : sd) += d * d;
+        }
+    }
+    __shared__ float red[2][256];
+    red[0][threadIdx.x] = sp;
+    red[1][threadIdx.x] = sd;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(acc, red[0][0]);
+        atomicAdd(acc + 1, red[1][0]);
+    }
+}
+
+// Gathers a mini-batch: rows idx[i] of the training states, network targets and RK4 exits (idx
+// from a counter-based hash of (seed, step, i)).
+__global__ void __launch_bounds__(256) gather_batch_kernel(const float* states, const float* targets,
+                                                           const float* fine, int n_data, unsigned long long seed,
+                                                           int step, int batch, float* bstates, float* btargets,
+                                                           float* bfine) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= batch) return;
     unsigned long long z = seed + 0x9E3779B97F4A7C15ull * ((unsigned long long)step * (unsigned long long)batch + i + 1);
@@ -81,10 +126,20 @@ __global__ void __launch_bounds__(256) gather_batch_kernel(const float* states, 
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z ^= z >> 31;
     const int r = (int)(z % (unsigned long long)n_data);
-    for (int k = 0; k < 6; ++k) bstates[6 * (size_t)i + k] = states[6 * (size_t)r + k];
-    const float inv = 1.0f / radius;
-    for (int k = 0; k < 3; ++k) btargets[6 * (size_t)i + k] = targets[6 * (size_t)r + k] * inv;
-    for (int k = 3; k < 6; ++k) btargets[6 * (size_t)i + k] = targets[6 * (size_t)r + k];
+    for (int k = 0; k < 6; ++k) {
+        bstates[6 * (size_t)i + k] = states[6 * (size_t)r + k];
+        btargets[6 * (size_t)i + k] = targets[6 * (size_t)r + k];
+        bfine[6 * (size_t)i + k] = fine[6 * (size_t)r + k];
+    }
+}
+
+// Exit-point error of a prediction against the RK4 exit, as the parity tests measure it:
+// max over x, y, z of |pred - exit| / max(1, |exit|).
+__device__ __forceinline__ float exit_error(float px, float py, float pz, const float* e) {
+    const float ex = fabsf(px - e[0]) / fmaxf(1.0f, fabsf(e[0]));
+    const float ey = fabsf(py - e[1]) / fmaxf(1.0f, fabsf(e[1]));
+    const float ez = fabsf(pz - e[2]) / fmaxf(1.0f, fabsf(e[2]));
+    return fmaxf(ex, fmaxf(ey, ez));
 }
 
 // FP32 output layer: out[i][o] = sum_k act[i][k] * W[k][o] + b[o] (bf16 activations, fp32
@@ -100,27 +155,29 @@ __global__ void __launch_bounds__(256) out_layer_kernel(const u16* act, const fl
     out[(size_t)i * LS_OUT + o] = s;
 }
 
-// Loss and output gradient.  Regression: mean squared error of the 6 scaled exit values.
-// Confidence: logistic loss against the label [exit-point error < tol], the error taken in
-// sphere units (|p_pred - p| * R).  Writes dOut [n][LS_OUT] (already / n) and per-row error.
-__global__ void __launch_bounds__(256) loss_kernel(const float* out, const float* target, int n, float radius,
-                                                   float tol, float conf_weight, float* dout, float* err,
-                                                   float* loss_sum) {
+// Loss and output gradient.  Regression: mean squared error of the 6 scaled targets.
+// Confidence: logistic loss against the label [exit-point error <= tol], the error being the
+// prediction's (base + output / scale_p) against the RK4 exit `fine`.  Writes dOut [n][LS_OUT]
+// (already / n) and the per-row error.
+__global__ void __launch_bounds__(256) loss_kernel(const float* out, const float* target, const float* fine, int n,
+                                                   float scale_p, float tol, float conf_weight, float* dout,
+                                                   float* err, float* loss_sum) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     float l = 0.0f;
     if (i < n) {
         const float* o = out + (size_t)i * LS_OUT;
         const float* y = target + 6 * (size_t)i;
-        float e2 = 0.0f;
+        const float* f = fine + 6 * (size_t)i;
+        float pe[3];
         for (int k = 0; k < 6; ++k) {
             float d = o[k] - y[k];
             l += d * d / 6.0f;
             dout[(size_t)i * LS_OUT + k] = 2.0f * d / (6.0f * n);
-            if (k < 3) e2 += d * d;
+            if (k < 3) pe[k] = f[k] + d / scale_p;  // the prediction: exit + (output - target) / scale
         }
-        const float e = sqrtf(e2) * radius;
+        const float e = exit_error(pe[0], pe[1], pe[2], f);
         err[i] = e;
-        const float label = e < tol ? 1.0f : 0.0f;
+        const float label = e <= tol ? 1.0f : 0.0f;
         const float c = o[6];
         const float s = 1.0f / (1.0f + __expf(-c));
         l += conf_weight * (fmaxf(c, 0.0f) - c * label + log1pf(__expf(-fabsf(c))));
@@ -219,17 +276,21 @@ __global__ void __launch_bounds__(256) adam_kernel(float* w, float* m, float* v,
 }
 
 // Surrogate verdict per entered ray: accept (write the exit point to the mapping) when the
-// confidence reaches the threshold; otherwise queue the ray for RK4.  Scaled outputs are
-// turned back into sphere units.
-__global__ void __launch_bounds__(256) accept_kernel(const float* out, const int* texel_of, int n, float radius,
-                                                     float conf_threshold, float* mapping, float* pred_exit,
-                                                     int* fallback, int* n_fallback, int* accepted) {
+// confidence reaches the threshold; otherwise queue the ray for RK4.  The exit point is the
+// base (coarse flow map, or 0) plus the output over the point scale.
+__global__ void __launch_bounds__(256) accept_kernel(const float* out, const float* base, const int* texel_of, int n,
+                                                     float scale_p, float conf_threshold, float* mapping,
+                                                     float* pred_exit, int* fallback, int* n_fallback,
+                                                     int* accepted) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* o = out + (size_t)i * LS_OUT;
     const float conf = 1.0f / (1.0f + __expf(-o[6]));
     const int texel = texel_of[i];
-    const float px = o[0] * radius, py = o[1] * radius, pz = o[2] * radius;
+    const float inv = 1.0f / scale_p;
+    const float* b = base ? base + 6 * (size_t)i : nullptr;
+    const float px = (b ? b[0] : 0.0f) + o[0] * inv, py = (b ? b[1] : 0.0f) + o[1] * inv,
+                pz = (b ? b[2] : 0.0f) + o[2] * inv;
     pred_exit[3 * (size_t)i] = px;
     pred_exit[3 * (size_t)i + 1] = py;
     pred_exit[3 * (size_t)i + 2] = pz;
@@ -264,7 +325,7 @@ __global__ void __launch_bounds__(256) compact_entered_kernel(const int* entered
 }
 
 // Rays re-judged by RK4 or sent to RK4: write the RK4 exit into the mapping; for re-judged
-// rays also record the surrogate's error (distance between predicted and RK4 exit points).
+// rays also record the surrogate's error against the RK4 exit (exit_error).
 __global__ void __launch_bounds__(256) apply_rk4_kernel(const int* list, int n, const float* exits,
                                                         const int* texel_of, const float* pred_exit, float* mapping,
                                                         float* err_out, int write_mapping) {
@@ -272,26 +333,36 @@ __global__ void __launch_bounds__(256) apply_rk4_kernel(const int* list, int n, 
     if (i >= n) return;
     const int r = list[i];
     const float* e = exits + 6 * (size_t)r;
-    if (err_out) {
-        const float dx = e[0] - pred_exit[3 * (size_t)r], dy = e[1] - pred_exit[3 * (size_t)r + 1],
-                    dz = e[2] - pred_exit[3 * (size_t)r + 2];
-        err_out[i] = sqrtf(dx * dx + dy * dy + dz * dz);
-    }
+    if (err_out)
+        err_out[i] = exit_error(pred_exit[3 * (size_t)r], pred_exit[3 * (size_t)r + 1], pred_exit[3 * (size_t)r + 2], e);
     if (write_mapping) reinterpret_cast<float4*>(mapping)[texel_of[r]] = make_float4(e[0], e[1], e[2], 1.0f);
 }
 
 // Accepted rays whose Lyapunov exponent (from the surrogate's own mapping) is above the
-// threshold are queued for RK4 as well.
-__global__ void __launch_bounds__(256) lyapunov_gate_kernel(const int* accepted, const int* texel_of, int n,
+// threshold are queued for RK4 as well, and no longer count as accepted.
+__global__ void __launch_bounds__(256) lyapunov_gate_kernel(int* accepted, const int* texel_of, int n,
                                                             const float* lyap, float threshold, int* list,
                                                             int* count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !accepted[i]) return;
     const float l = lyap[texel_of[i]];
     if (l > threshold) {
+        accepted[i] = 0;
         int slot = atomicAdd(count, 1);
         list[slot] = i;
     }
+}
+
+// Per texel: 1 where the frame kept the surrogate's exit (mask cleared beforehand).
+__global__ void __launch_bounds__(256) accepted_mask_kernel(const int* accepted, const int* texel_of, int n,
+                                                            unsigned char* mask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && accepted[i]) mask[texel_of[i]] = 1;
+}
+
+__global__ void __launch_bounds__(256) fill_u8_kernel(unsigned char* p, int n, unsigned char value) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
 }
 
 __global__ void __launch_bounds__(256) fill_i32_kernel(int* p, int n, int value) {
@@ -312,23 +383,42 @@ hipError_t sg_gemm(const lm::GemmParams& P, int splits, hipStream_t s) {
 
 #define LS_GRID(n) dim3(((n) + 255) / 256), dim3(256)
 
+hipError_t sg_mlp_pack(const float* W, int K, u16* dst, hipStream_t s) {
+    hipLaunchKernelGGL(mlp_pack_kernel, LS_GRID((K / 16) * 8 * 64), 0, s, W, K, dst);
+    return hipGetLastError();
+}
+hipError_t sg_mlp_infer(const lm::MlpInferParams& P, hipStream_t s) {
+    if (P.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3((P.n + LM_FUSED_ROWS - 1) / LM_FUSED_ROWS), dim3(256), 0, s, P);
+    return hipGetLastError();
+}
 hipError_t sg_features(const ls::SurrogateIO& S, hipStream_t s) {
     hipLaunchKernelGGL(features_kernel, LS_GRID(S.n), 0, s, S);
     return hipGetLastError();
 }
-hipError_t sg_gather(const float* states, const float* targets, int n_data, unsigned long long seed, int step,
-                     int batch, float radius, float* bstates, float* btargets, hipStream_t s) {
-    hipLaunchKernelGGL(gather_batch_kernel, LS_GRID(batch), 0, s, states, targets, n_data, seed, step, batch, radius,
-                       bstates, btargets);
+hipError_t sg_make_targets(const float* fine, const float* base, int n, float scale_p, float scale_d, float* tgt,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(make_targets_kernel, LS_GRID(6 * n), 0, s, fine, base, n, scale_p, scale_d, tgt);
+    return hipGetLastError();
+}
+hipError_t sg_residual_sumsq(const float* fine, const float* base, int n, float* acc, hipStream_t s) {
+    hipLaunchKernelGGL(residual_sumsq_kernel, LS_GRID(n), 0, s, fine, base, n, acc);
+    return hipGetLastError();
+}
+hipError_t sg_gather(const float* states, const float* targets, const float* fine, int n_data, unsigned long long seed,
+                     int step, int batch, float* bstates, float* btargets, float* bfine, hipStream_t s) {
+    hipLaunchKernelGGL(gather_batch_kernel, LS_GRID(batch), 0, s, states, targets, fine, n_data, seed, step, batch,
+                       bstates, btargets, bfine);
     return hipGetLastError();
 }
 hipError_t sg_out_layer(const u16* act, const float* W, const float* b, int n, float* out, hipStream_t s) {
     hipLaunchKernelGGL(out_layer_kernel, LS_GRID(n * LS_OUT), 0, s, act, W, b, n, out);
     return hipGetLastError();
 }
-hipError_t sg_loss(const float* out, const float* target, int n, float radius, float tol, float conf_weight,
-                   float* dout, float* err, float* loss_sum, hipStream_t s) {
-    hipLaunchKernelGGL(loss_kernel, LS_GRID(n), 0, s, out, target, n, radius, tol, conf_weight, dout, err, loss_sum);
+hipError_t sg_loss(const float* out, const float* target, const float* fine, int n, float scale_p, float tol,
+                   float conf_weight, float* dout, float* err, float* loss_sum, hipStream_t s) {
+    hipLaunchKernelGGL(loss_kernel, LS_GRID(n), 0, s, out, target, fine, n, scale_p, tol, conf_weight, dout, err,
+                       loss_sum);
     return hipGetLastError();
 }
 hipError_t sg_out_layer_back(const float* dout, const float* W, const float* z, int n, u16* dz, hipStream_t s) {
@@ -352,10 +442,10 @@ hipError_t sg_adam(float* w, float* m, float* v, const float* g, int K, int N, f
     hipLaunchKernelGGL(adam_kernel, LS_GRID(K * N), 0, s, w, m, v, g, K, N, lr, b1, b2, eps, bc1, bc2, wt, ldt);
     return hipGetLastError();
 }
-hipError_t sg_accept(const float* out, const int* texel_of, int n, float radius, float thr, float* mapping,
-                     float* pred_exit, int* fallback, int* n_fallback, int* accepted, hipStream_t s) {
-    hipLaunchKernelGGL(accept_kernel, LS_GRID(n), 0, s, out, texel_of, n, radius, thr, mapping, pred_exit, fallback,
-                       n_fallback, accepted);
+hipError_t sg_accept(const float* out, const float* base, const int* texel_of, int n, float scale_p, float thr,
+                     float* mapping, float* pred_exit, int* fallback, int* n_fallback, int* accepted, hipStream_t s) {
+    hipLaunchKernelGGL(accept_kernel, LS_GRID(n), 0, s, out, base, texel_of, n, scale_p, thr, mapping, pred_exit,
+                       fallback, n_fallback, accepted);
     return hipGetLastError();
 }
 hipError_t sg_compact_entered(const int* entered, const float* entry_states, int n, float* states, int* texel_of,
@@ -369,7 +459,13 @@ hipError_t sg_apply_rk4(const int* list, int n, const float* exits, const int* t
                        write_mapping);
     return hipGetLastError();
 }
-hipError_t sg_lyapunov_gate(const int* accepted, const int* texel_of, int n, const float* lyap, float thr, int* list,
+hipError_t sg_accepted_mask(const int* accepted, const int* texel_of, int n, int npx, unsigned char* mask,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(fill_u8_kernel, LS_GRID(npx), 0, s, mask, npx, (unsigned char)0);
+    if (n > 0) hipLaunchKernelGGL(accepted_mask_kernel, LS_GRID(n), 0, s, accepted, texel_of, n, mask);
+    return hipGetLastError();
+}
+hipError_t sg_lyapunov_gate(int* accepted, const int* texel_of, int n, const float* lyap, float thr, int* list,
                             int* count, hipStream_t s) {
     hipLaunchKernelGGL(lyapunov_gate_kernel, LS_GRID(n), 0, s, accepted, texel_of, n, lyap, thr, list, count);
     return hipGetLastError();

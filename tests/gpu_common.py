@@ -169,3 +169,15 @@ def compare(gpu: "L.Frame", ref: dict, ref64: dict, label: str, lam_gpu=None):
     info["exit_pass"] = bool(info["exit_ok"] >= HIT_MIN_AGREE or info["exit_agree"] >= bar)
     print(info, flush=True)
     return info
+
+
+def entry_states(n, seed, radius=2.0, inset=0.01):
+    """Entry states as the surrogate's training set draws them: p uniform on the sphere, pulled
+    inside by up to `inset`, q uniform over the directions pointing into it."""
+    rng = np.random.default_rng(seed)
+    p = rng.normal(size=(n, 3))
+    p *= (radius - inset * rng.uniform(size=(n, 1))) / np.linalg.norm(p, axis=1, keepdims=True)
+    q = rng.normal(size=(n, 3))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    q[(p * q).sum(1) > 0] *= -1.0
+    return np.concatenate([p, q], 1).astype(np.float32)

@@ -384,3 +384,30 @@ def test_wgsl_integer_division_is_total(view):
     fr = run_gpu(view, cam, L.Settings.make(), ig, body, np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
     v = (fr.mapping[..., :3].astype(np.float64) - [0.0, 0.0, 3.0]) / (fr.steps[..., None] * 0.1)
     np.testing.assert_allclose(v, np.broadcast_to([1.0, 0.5, 2.0], v.shape), atol=1e-4)
+
+
+@pytest.mark.parametrize("field", ["lens", "hot_air"])
+def test_outline_search_at_4k(field):
+    """The outline search on the GPU's own 3840x2160 exponents buffer (row stride H, rows
+    aliasing since W > H) finds exactly the oracle search's pixels; the buffer holds highlighted
+    texels (exponent >= 0.8) for the search to find.  Cornell box, h 0.005 (the search reads only
+    the exponents)."""
+    wl = configs.STRESS_4K
+    v, f = scenes.cornell_box_centered()
+    with L.MainView(wl.width, wl.height) as big:
+        big.update_model(v, f)
+        big.update_camera(wl.camera())
+        big.update_settings(wl.settings())
+        ig = wl.integrator()
+        ig.h_initial = 0.005
+        big.set_integrator(ig)
+        big.set_field(field)
+        print(f"[case] 4K outline search {field}", flush=True)
+        big.render()          # exponents of this frame ...
+        exps = big.read_exponents()
+        big.render()          # ... are what the next frame's outline search reads
+        got = big.read_outline()
+    n_hi = int((exps >= 0.8).sum())
+    print(f"{field}: {n_hi} exponents >= 0.8, found {got[:, 2].tolist()}", flush=True)
+    np.testing.assert_array_equal(got, O.outline_search(exps, wl.width, wl.height))
+    assert n_hi > 0 and got[:, 2].sum() > 0
