@@ -207,6 +207,8 @@ typedef struct linon_batch_tuning {
  * field (MirageSphericalSigmoid, :100).  device = HIP device ordinal. */
 linon_status linon_create(linon_context** out, int device, uint32_t width, uint32_t height);
 void linon_destroy(linon_context* ctx);
+/* The context's last error message; with ctx = NULL, the message of the calling thread's last
+ * failed call that takes no context (linon_flowmap_noalloc), or "null context". */
 const char* linon_last_error(const linon_context* ctx);
 int linon_abi_version(void);
 
@@ -344,6 +346,16 @@ linon_status linon_surrogate_load(linon_context* ctx, const void* data, size_t b
  * the RK4 iterations.  Fields: swirl, lens, hot air, warp, identity, spherical sigmoid mirage. */
 linon_status linon_flowmap(int device, int field, int math, const linon_integrator* integrator, float field_weight,
                            const float* states, int n, float* exits, int32_t* steps);
+/* linon_flowmap without any device allocation (no counterpart in the reference): for callers
+ * that cannot allocate device memory, such as tight serving loops and harnesses that forbid
+ * hipMalloc.  Same arguments, kernels and results as linon_flowmap, but the states, exits and
+ * iteration counts are staged in a fixed device array the library holds per device from the
+ * first call on it until the process ends, so n is at most LINON_FLOWMAP_NOALLOC_MAX; larger n
+ * is rejected with LINON_ERR_ARGUMENT and a message (linon_last_error(NULL)).  Calls that
+ * overlap on one device share that array and race: serialise them. */
+#define LINON_FLOWMAP_NOALLOC_MAX 65536
+linon_status linon_flowmap_noalloc(int device, int field, int math, const linon_integrator* integrator,
+                                   float field_weight, const float* states, int n, float* exits, int32_t* steps);
 /* Diagnostics: the reciprocal-based divisions of the IEEE build against the IEEE division, bit
  * for bit: the reciprocal for every mantissa at exponents -126..125 (both signs); the
  * shared-denominator division (lw::div3_rn) for every numerator mantissa at three exponents

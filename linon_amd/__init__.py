@@ -280,6 +280,8 @@ SIGNATURES = {
     "linon_surrogate_infer": (C.c_int, [_P, _P, C.c_int, _P, C.c_int]),
     "linon_read_surrogate_accepted": (C.c_int, [_P, _P, C.c_size_t]),
     "linon_flowmap": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(Integrator), C.c_float, _P, C.c_int, _P, _P]),
+    "linon_flowmap_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(Integrator), C.c_float, _P, C.c_int, _P,
+                                        _P]),
 }
 
 
@@ -603,18 +605,25 @@ class MainView:
                      self.read_steps(), self.read_lyapunov(), self.read_exponents(), self.counters())
 
 
-def flowmap(field, integrator: Integrator, states, field_weight: float = 1.0, math: int = 0, device: int = 0):
-    """Flow map through the bounding sphere on the GPU (linon_flowmap): (n, 6) entry states
-    (p, q) -> ((n, 6) end point and final direction, (n,) RK4 iterations)."""
+FLOWMAP_NOALLOC_MAX = 65536   # LINON_FLOWMAP_NOALLOC_MAX
+
+
+def flowmap(field, integrator: Integrator, states, field_weight: float = 1.0, math: int = 0, device: int = 0,
+            noalloc: bool = False):
+    """Flow map through the bounding sphere on the GPU (linon_flowmap, or with noalloc
+    linon_flowmap_noalloc, which stages in the library's fixed device array and takes at most
+    FLOWMAP_NOALLOC_MAX states): (n, 6) entry states (p, q) -> ((n, 6) end point and final
+    direction, (n,) RK4 iterations)."""
     lib = load_library()
     fid = FIELDS[field] if isinstance(field, str) else int(field)
     x = np.ascontiguousarray(states, np.float32).reshape(-1, 6)
     out = np.zeros_like(x)
     steps = np.zeros(len(x), np.int32)
-    st = lib.linon_flowmap(device, fid, math, C.byref(integrator), field_weight, _ptr(x), len(x), _ptr(out),
-                           _ptr(steps))
+    fn = lib.linon_flowmap_noalloc if noalloc else lib.linon_flowmap
+    st = fn(device, fid, math, C.byref(integrator), field_weight, _ptr(x), len(x), _ptr(out), _ptr(steps))
     if st != LINON_OK:
-        raise LinonError(st, "linon_flowmap failed")
+        msg = lib.linon_last_error(None) if noalloc else None
+        raise LinonError(st, msg.decode() if msg else "linon_flowmap failed")
     return out, steps
 
 

@@ -238,6 +238,12 @@ linon_status Context::render_with(const linon_integrator& ig, hipStream_t s) {
 
 bool Context::wgsl_math_for(int field) const { return wgsl_math_default(math, field); }
 
+namespace {
+thread_local std::string g_unbound_error;
+}
+void set_unbound_error(const std::string& msg) { g_unbound_error = msg; }
+const std::string& unbound_error() { return g_unbound_error; }
+
 bool wgsl_math_default(int math, int field) {
     if (math == LINON_MATH_IEEE) return false;
     if (math == LINON_MATH_WGSL) return true;
@@ -409,7 +415,7 @@ void linon_destroy(linon_context* ctx) {
 
 const char* linon_last_error(const linon_context* ctx) {
     const Context* c = reinterpret_cast<const Context*>(ctx);
-    if (!c) return "null context";
+    if (!c) return linon::unbound_error().empty() ? "null context" : linon::unbound_error().c_str();
     return c->err.c_str();
 }
 

@@ -187,6 +187,10 @@ struct Context {
 void fill_step(StepParams& I, const linon_integrator& ig, float field_weight);
 // The kernel build (WGSL accuracy or IEEE) that linon_math value `math` selects for a field.
 bool wgsl_math_default(int math, int field);
+// Error message of the calling thread's last failed entry point that takes no context
+// (linon_last_error(NULL) returns it).
+void set_unbound_error(const std::string& msg);
+const std::string& unbound_error();
 void fill_camera(CameraParams& C, const linon_camera& cam, uint32_t w, uint32_t h);
 
 }  // namespace linon

@@ -158,3 +158,17 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=120, cwd=root)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_flowmap_noalloc_rejects_more_than_its_staging():
+    """linon_flowmap_noalloc checks n against its fixed device staging before it touches a
+    device (so this runs without a GPU) and leaves a message for linon_last_error(NULL)."""
+    lib = L.load_library()
+    ig = L.Integrator.make(mode=L.MODE_ESCAPE)
+    x = np.zeros((1, 6), np.float32)
+    st = lib.linon_flowmap_noalloc(0, L.FIELDS["swirl"], 0, C.byref(ig), 1.0, C.c_void_p(x.ctypes.data),
+                                   L.FLOWMAP_NOALLOC_MAX + 1, C.c_void_p(x.ctypes.data), None)
+    assert st == 1
+    msg = lib.linon_last_error(None).decode()
+    assert str(L.FLOWMAP_NOALLOC_MAX) in msg and "staging" in msg
+    assert lib.linon_flowmap_noalloc(0, L.FIELDS["swirl"], 0, None, 1.0, None, 1, None, None) == 1
