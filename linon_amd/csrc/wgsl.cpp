@@ -11,6 +11,7 @@
 // clamp keeps WGSL's (e, low, high) argument order.
 #include "wgsl.hpp"
 
+#include <algorithm>
 #include <cctype>
 #include <cstring>
 #include <sstream>
@@ -282,13 +283,18 @@ class Parser {
     std::vector<std::vector<std::pair<std::string, Ty>>> scopes_;
     bool last_returns_ = false;  // did the statement just parsed always return?
 
-    const Token& cur() const { return t_[p_]; }
+    // the token stream ends in one End token, which neither cur() nor take() ever moves past
+    const Token& cur() const { return t_[std::min(p_, t_.size() - 1)]; }
     const Token& look(size_t k) const { return t_[std::min(p_ + k, t_.size() - 1)]; }
     bool at_end() const { return cur().kind == Tok::End; }
     bool is(const char* s) const { return cur().kind == Tok::Punct && cur().text == s; }
     bool is_kw(const char* s) const { return cur().kind == Tok::Ident && cur().text == s; }
     [[noreturn]] void fail(const std::string& m) const { throw TranslateError(m, cur().line, cur().col); }
-    Token take() { return t_[p_++]; }
+    Token take() {
+        Token t = cur();
+        if (p_ + 1 < t_.size()) ++p_;
+        return t;
+    }
     void expect(const char* s) {
         if (!is(s)) fail(std::string("expected '") + s + "', found '" + (at_end() ? "end of body" : cur().text) + "'");
         ++p_;

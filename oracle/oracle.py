@@ -34,13 +34,16 @@ transcendentals and FMA contraction), which no file of the reference pins.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "liblinon_oracle.so"
+# LINON_ORACLE_SANITIZE=1 selects the AddressSanitizer/UndefinedBehaviorSanitizer builds
+_SAN = "_san" if os.environ.get("LINON_ORACLE_SANITIZE") == "1" else ""
+LIB_PATH = HERE / f"liblinon_oracle{_SAN}.so"
 
 
 class CameraIn(C.Structure):
@@ -62,11 +65,11 @@ class IntegratorIn(C.Structure):
 
 _lib = None
 _lib_fma = None
-LIB_FMA_PATH = HERE / "liblinon_oracle_fma.so"
+LIB_FMA_PATH = HERE / f"liblinon_oracle_fma{_SAN}.so"
 
 
 def build() -> Path:
-    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    subprocess.run(["make", "-s", "-C", str(HERE)] + (["sanitize"] if _SAN else []), check=True)
     return LIB_PATH
 
 
