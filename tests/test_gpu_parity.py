@@ -389,9 +389,11 @@ def test_wgsl_integer_division_is_total(view):
 @pytest.mark.parametrize("field", ["lens", "hot_air"])
 def test_outline_search_at_4k(field):
     """The outline search on the GPU's own 3840x2160 exponents buffer (row stride H, rows
-    aliasing since W > H) finds exactly the oracle search's pixels; the buffer holds highlighted
-    texels (exponent >= 0.8) for the search to find.  Cornell box, h 0.005 (the search reads only
-    the exponents)."""
+    aliasing since W > H) finds exactly the oracle search's pixels.  Lens's buffer holds
+    highlighted texels (exponent >= 0.8) for the search to find; hot air's, at this pixel pitch,
+    holds none (its largest exponent is printed), so every search comes back empty and the
+    sample rays start at texel (0, 0), as in the reference.  Cornell box, h 0.005 (the search
+    reads only the exponents)."""
     wl = configs.STRESS_4K
     v, f = scenes.cornell_box_centered()
     with L.MainView(wl.width, wl.height) as big:
@@ -408,6 +410,8 @@ def test_outline_search_at_4k(field):
         big.render()          # ... are what the next frame's outline search reads
         got = big.read_outline()
     n_hi = int((exps >= 0.8).sum())
-    print(f"{field}: {n_hi} exponents >= 0.8, found {got[:, 2].tolist()}", flush=True)
+    print(f"{field}: {n_hi} exponents >= 0.8 (max {exps.max():.3f}), found {got[:, 2].tolist()}", flush=True)
     np.testing.assert_array_equal(got, O.outline_search(exps, wl.width, wl.height))
-    assert n_hi > 0 and got[:, 2].sum() > 0
+    assert (got[:, 2].sum() > 0) == (n_hi > 0)
+    if field == "lens":
+        assert n_hi > 0
