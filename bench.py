@@ -108,6 +108,51 @@ def frame_accuracy(wl, verts, faces, frames: dict, sample: np.ndarray, f32: dict
     return out
 
 
+def sample_accuracy(camera, settings, integ, field: str, verts, faces, W: int, H: int, mapping, hits, n: int,
+                    seed: int = 0) -> dict:
+    """A GPU frame (texel-space mapping and hit faces) at n random pixels against the FP32
+    oracle, and both against the FP64 oracle: hit agreement and the exit-point error (max over
+    x, y, z of |exit - ref| / max(1, |ref|)).  Runs outside every timed region."""
+    from oracle import oracle as O
+
+    threads = max(1, min(16, O.threads_available()))
+    c = O.as_oracle_struct(camera, O.CameraIn)
+    s = O.as_oracle_struct(settings, O.SettingsIn)
+    i = O.as_oracle_struct(integ, O.IntegratorIn)
+    rng = np.random.default_rng(seed)
+    px = np.stack([rng.integers(0, W, n), rng.integers(0, H, n)], 1).astype(np.int32)
+    rows, cols = H - 1 - px[:, 1], px[:, 0]
+    g_map, g_hit = mapping[rows, cols], hits[rows, cols]
+    r32 = O.render_pixels(c, s, i, L.FIELDS[field], verts, faces, W, H, px, threads)
+    r64 = O.render_pixels(c, s, i, L.FIELDS[field], verts, faces, W, H, px, threads, precision="f64")
+    return {"pixels": n, "hit_agree_vs_f32_oracle": float((g_hit == r32["hit"]).mean()),
+            "gpu_vs_f32_oracle": _quantiles(_rel_err(g_map, r32["mapping"])),
+            "vs_f64_oracle": {"gpu": _quantiles(_rel_err(g_map, r64["mapping"])),
+                              "f32_oracle": _quantiles(_rel_err(r32["mapping"], r64["mapping"]))}}
+
+
+def bench_plain_256(reps: int = 5) -> dict:
+    """The reference's CPU-runnable case: 256x256, Cornell box, swirl, h 0.01 (max 1000 RK4
+    iterations), no overlay; device time of `reps` renders and the accuracy at 2048 pixels."""
+    wl = configs.PLAIN_256_SWIRL
+    v, f = wl.mesh()
+    with L.MainView(wl.width, wl.height) as view:
+        view.update_model(v, f)
+        configs.setup_view(view, wl, "swirl")
+        view.render()
+        c = view.counters()
+        view.reset_timings()
+        for _ in range(reps):
+            view.render()
+        rms, pms = view.timings()
+        fr = view.read_frame()
+    r_ms = float(np.mean(rms))
+    acc = sample_accuracy(wl.camera(), wl.settings(), wl.integrator(), "swirl", v, f, wl.width, wl.height,
+                          fr.mapping, fr.hit_faces, 2048, seed=1)
+    return {"workload": wl.name, "raycast_ms": r_ms, "frame_ms": float(np.mean(pms)), "rk4_steps": c["rk4_steps"],
+            "gsteps_per_s": c["rk4_steps"] / (r_ms * 1e-3) / 1e9, "accuracy": acc}
+
+
 def cpu_baseline(wl, verts, faces, budget_s: float, frames: dict | None = None) -> dict:
     """FP32 CPU restatement on a strided sample of each field's pixels.  With the GPU's frames
     given, the same sample also yields the frame_accuracy() report (outside the timed region)."""
@@ -286,6 +331,8 @@ def main() -> int:
     }
     view.close()
     extra = {}
+    log("plain 256x256 ...")
+    extra["plain_256_swirl"] = bench_plain_256()
     if not a.skip_batch:
         log("batched views ...")
         extra["batched_views"] = bench_batch(verts, faces, a.batch_replays)
@@ -381,6 +428,10 @@ def bench_stress() -> dict:
                                   # highlighted texels (exponent >= 0.8) in the buffer they searched
                                   "outline_rays_found": int(view.read_outline()[:, 2].sum()),
                                   "searched_exponents_ge_0.8": searched_hi}
+        fr = view.read_frame()
+        out["per_field"][name]["accuracy"] = sample_accuracy(wl.camera(), wl.settings(), wl.integrator(), name,
+                                                             verts, faces, wl.width, wl.height, fr.mapping,
+                                                             fr.hit_faces, 64, seed=2)
         log(f"stress {name}: {rms[0]:.1f} ms")
     # both fields' frames captured into one HIP graph on a caller's stream and replayed (the
     # graph keeps each capture's field; the frames share the output buffers)
@@ -404,6 +455,20 @@ def bench_batch(verts, faces, replays: int) -> dict:
     view.render_batch()
     view.synchronize()
     c = view.counters()
+    # accuracy of three views (one per field kind among the first ones) at 256 pixels each
+    accuracy = {}
+    seen = set()
+    for i, bv in enumerate(views):
+        if bv.field in seen or len(accuracy) == 3:
+            continue
+        seen.add(bv.field)
+        fr = view.read_batch_view(i)
+        ig = configs.batch_integrator(max_steps=8000)
+        ig.h_initial = bv.h_initial
+        name = {v: k for k, v in L.FIELDS.items()}[bv.field]
+        accuracy[f"view{i}_{name}_{bv.width}x{bv.height}_h{bv.h_initial:g}"] = sample_accuracy(
+            bv.camera, L.Settings.make(field_weight=bv.field_weight), ig, name, verts, faces, bv.width, bv.height,
+            fr.mapping, fr.hit_faces, 256, seed=3)
     view.reset_timings()
     t0 = time.perf_counter()
     for _ in range(replays):
@@ -433,7 +498,8 @@ def bench_batch(verts, faces, replays: int) -> dict:
             "wall_ms": wall * 1e3, "gsteps_per_s": c["rk4_steps"] / (dev_ms * 1e-3) / 1e9,
             "mrays_per_s": rays / (dev_ms * 1e-3) / 1e6, "plain_kernel_sequential_ms": plain_ms,
             "speedup_vs_plain_sequential": plain_ms / dev_ms if dev_ms > 0 else None,
-            "tuning": tuning.as_dict(), "autotune_full_size_ms": tuned_ms, "autotune_seconds": tune_s}
+            "tuning": tuning.as_dict(), "autotune_full_size_ms": tuned_ms, "autotune_seconds": tune_s,
+            "accuracy": accuracy}
 
 
 def bench_surrogate(steps: int, fields, wl=configs.OVERLAY_1080P, train: bool = False) -> dict:

@@ -83,6 +83,9 @@ struct Surrogate {
     unsigned char* f_mask = nullptr;  // per texel: the frame kept the surrogate's exit
     linon_surrogate_stats train_stats{};
     linon_surrogate_stats frame_stats{};
+    hipEvent_t ev[7] = {};      // frame phase boundaries
+    bool frame_timed = false;   // the last frame recorded them (it was not captured)
+    bool stats_stale = false;   // frame_stats not yet read back for the last frame
     void release();
 };
 
@@ -170,6 +173,7 @@ struct Context {
     linon_status surrogate_eval(const float* states, const float* targets, const float* fine, int n,
                                 std::vector<float>& err);
     linon_status surrogate_frame(hipStream_t s);
+    linon_status surrogate_collect_stats();
     linon_status surrogate_save(void* out, size_t capacity, size_t* needed);
     linon_status surrogate_pack();  // refreshes d_wpack from the master weights
     linon_status surrogate_infer(const float* states, int n, float* out, bool fused);

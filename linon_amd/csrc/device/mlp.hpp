@@ -47,16 +47,30 @@ struct GemmParams {
     int ldc;
 };
 
-__device__ __forceinline__ u16 f2bf(float x) {  // round to nearest even
-    unsigned u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (u16)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (u16)(u >> 16);
+// float -> bf16 bits, rounded to nearest even (one v_cvt_pk_bf16_f32 on gfx950; NaN stays NaN)
+__device__ __forceinline__ u16 f2bf(float x) {
+    const __bf16 b = (__bf16)x;
+    u16 u;
+    __builtin_memcpy(&u, &b, 2);
+    return u;
+}
+// two floats -> packed bf16 pair (a in the low half), one conversion instruction
+__device__ __forceinline__ unsigned f2bf2(float a, float b) {
+    typedef float f32x2v __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+    const bf16x2v v = __builtin_convertvector((f32x2v){a, b}, bf16x2v);
+    unsigned u;
+    __builtin_memcpy(&u, &v, 4);
+    return u;
 }
 __device__ __forceinline__ float bf2f(u16 b) { return __uint_as_float(((unsigned)b) << 16); }
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative through the hardware reciprocal (v_rcp_f32, 1 ulp): a network
+// needs no correctly rounded division, and every kernel (training GEMM epilogues, the fused
+// inference) evaluates these same expressions.
+__device__ __forceinline__ float sigmoid_fast(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigmoid_fast(z); }
 __device__ __forceinline__ float silu_grad(float z) {
-    float s = 1.0f / (1.0f + __expf(-z));
+    float s = sigmoid_fast(z);
     return s * (1.0f + z * (1.0f - s));
 }
 

@@ -115,7 +115,8 @@ struct FlowmapParams {
     StepParams step;        // escape mode, need_mapping, no mesh
     const float* states;    // [n][6] entry states (p, q)
     const int* index;       // optional: states[index[i]] (compacted subsets); null = identity
-    int n;
+    int n;                  // states (or list entries); with n_dev, the launch's capacity
+    const int* n_dev;       // optional: the count on the device (a frame's compacted lists)
     float* exits;           // [n or max index][6] exit point and direction, at the state's slot
     int* steps;             // optional, RK4 iterations per state slot (added to what is there)
 };
@@ -125,7 +126,7 @@ struct FlowmapParams {
 template <class F>
 __device__ void flowmap_body(const FlowmapParams& P) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
+    if (i >= (P.n_dev ? *P.n_dev : P.n)) return;
     const int slot = P.index ? P.index[i] : i;
     const float* s = P.states + 6 * (size_t)slot;
     lk::RayState r;

@@ -69,6 +69,8 @@ void Surrogate::release() {
                     f_accepted, f_list, f_exits, f_errs, f_feat, f_act0, f_act1, f_out, f_steps, f_base, f_mask};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
     *this = Surrogate();
 }
 
@@ -500,19 +502,28 @@ linon_status Context::surrogate_eval(const float* states, const float* targets, 
 
 // One surrogate frame of the current view (no mesh: the mapping texture and the overlay):
 //   1. RK4 from the camera to the bounding sphere (rays that never enter end there);
-//   2. the MLP on the rays that entered (compacted), accepting confident predictions;
+//   2. compaction of the rays that entered; the coarse flow map of their entry states (the
+//      multi-fidelity form) and the MLP, accepting confident predictions;
 //   3. RK4 from the entry state for rays below the confidence threshold;
 //   4. the overlay's Lyapunov exponent on that mapping; accepted rays above the threshold go
 //      to RK4 as well, and the overlay runs again;
 //   5. re-judging: a hash-spread sample of the accepted rays is also run with RK4 and the
 //      surrogate's exit-point error on it recorded.
+// The compacted lists' lengths stay on the device: every launch is sized for the whole image
+// and reads its count there, so the frame never waits for the host and can be captured into a
+// HIP graph.  The statistics (counts, re-judged errors) are read when they are asked for; the
+// phase timings are recorded when the stream is not being captured.
 linon_status Context::surrogate_frame(hipStream_t s) {
     Surrogate& S = sg;
     if (!S.trained) return fail(LINON_ERR_STATE, "render_surrogate: no trained surrogate (linon_surrogate_train)");
     if (!d_color) return fail(LINON_ERR_STATE, "render_surrogate: no image buffers (the last linon_resize failed)");
     const int W = (int)width, H = (int)height, npx = W * H;
     LS_CHECK(this, hipSetDevice(device));
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    LS_CHECK(this, hipStreamIsCapturing(s, &cap));
+    const bool capturing = cap != hipStreamCaptureStatusNone;
     if (S.frame_px != npx) {
+        if (capturing) return fail(LINON_ERR_STATE, "render_surrogate: render once at this size before capturing");
         LS_CHECK(this, dalloc(S.f_entry, (size_t)npx * 6));
         LS_CHECK(this, dalloc(S.f_entered, (size_t)npx));
         LS_CHECK(this, dalloc(S.f_states, (size_t)npx * 6));
@@ -528,15 +539,24 @@ linon_status Context::surrogate_frame(hipStream_t s) {
         LS_CHECK(this, dalloc(S.f_mask, (size_t)npx));
         S.frame_px = npx;
     }
-    hipEvent_t e[7];  // e[6]: after the coarse flow map, inside the MLP phase
-    for (auto& x : e) LS_CHECK(this, hipEventCreate(&x));
-    auto destroy = [&]() {
-        for (auto& x : e) (void)hipEventDestroy(x);
-    };
+    if (!S.ev[0])
+        for (auto& x : S.ev) LS_CHECK(this, hipEventCreate(&x));
+    hipEvent_t* e = S.ev;  // e[6]: after the coarse flow map, inside the MLP phase
+    auto mark = [&](int k) { return capturing ? hipSuccess : hipEventRecord(e[k], s); };
+    // device counts: [0] entered, [1] confidence fallback, [2] Lyapunov fallback, [3] re-judged
+    int* const n_in = S.f_count;
+    int* const n_fb = S.f_count + 1;
+    int* const n_ly = S.f_count + 2;
+    int* const n_rj = S.f_count + 3;
     const StepParams I = surrogate_step_params();
     const bool q_is_v0 = S.cfg.field == LINON_FIELD_SWIRL || S.cfg.field == LINON_FIELD_WARP ||
                          (S.cfg.field >= 5 && S.cfg.field <= 7);
-    LS_CHECK(this, hipEventRecord(e[0], s));
+    const bool wgsl = wgsl_math_for(S.cfg.field);
+    auto flowmap = [&](const ls::FlowmapParams& F) {
+        return wgsl ? linon_wgslmath::sg_flowmap(S.cfg.field, F, s) : sg_flowmap(S.cfg.field, F, s);
+    };
+    LS_CHECK(this, mark(0));
+    LS_CHECK(this, sg_fill_i32(S.f_count, 4, 0, s));
     // 1. approach
     ls::ApproachParams A;
     std::memset(&A, 0, sizeof(A));
@@ -549,63 +569,50 @@ linon_status Context::surrogate_frame(hipStream_t s) {
     A.mapping = d_mapping;
     A.steps = d_steps;
     A.q_is_v0 = q_is_v0 ? 1 : 0;
-    LS_CHECK(this, wgsl_math_for(S.cfg.field) ? linon_wgslmath::sg_approach(S.cfg.field, A, s)
-                                                : sg_approach(S.cfg.field, A, s));
-    LS_CHECK(this, sg_fill_i32(S.f_count, 1, 0, s));
-    LS_CHECK(this, sg_compact_entered(S.f_entered, S.f_entry, npx, S.f_states, S.f_texel, S.f_count, s));
-    int n_in = 0;
-    LS_CHECK(this, hipMemcpyAsync(&n_in, S.f_count, 4, hipMemcpyDeviceToHost, s));
-    LS_CHECK(this, hipStreamSynchronize(s));
-    LS_CHECK(this, hipEventRecord(e[1], s));
-    // 2. MLP (over the coarse flow map of the entered rays, when the network learnt a residual)
-    int n_fb = 0;
+    LS_CHECK(this, wgsl ? linon_wgslmath::sg_approach(S.cfg.field, A, s) : sg_approach(S.cfg.field, A, s));
+    LS_CHECK(this, sg_compact_entered(S.f_entered, S.f_entry, npx, S.f_states, S.f_texel, n_in, s));
+    LS_CHECK(this, mark(1));
+    // 2. coarse flow map (multi-fidelity form), then the MLP in one fused launch
     const bool coarse = S.cfg.coarse_h > 0.0f;
-    if (n_in > 0) {
-        if (coarse) {
-            ls::FlowmapParams Fc;
-            std::memset(&Fc, 0, sizeof(Fc));
-            Fc.step = surrogate_coarse_step_params();
-            Fc.states = S.f_states;
-            Fc.n = n_in;
-            Fc.exits = S.f_base;
-            LS_CHECK(this, wgsl_math_for(S.cfg.field) ? linon_wgslmath::sg_flowmap(S.cfg.field, Fc, s)
-                                                        : sg_flowmap(S.cfg.field, Fc, s));
-        }
-        LS_CHECK(this, hipEventRecord(e[6], s));
-        // features, the four hidden layers and the output layer in one fused launch
-        lm::MlpInferParams M;
-        std::memset(&M, 0, sizeof(M));
-        M.states = S.f_states;
-        M.n = n_in;
-        M.radius = S.cfg.radius;
-        M.freqs = S.d_freqs;
-        M.wpack = S.d_wpack;
-        for (int l = 0; l < 4; ++l) M.bias[l] = S.b[l];
-        M.w_out = S.W[4];
-        M.b_out = S.b[4];
-        M.out = S.f_out;
-        LS_CHECK(this, sg_mlp_infer(M, s));
-        LS_CHECK(this, sg_fill_i32(S.f_count, 1, 0, s));
-        LS_CHECK(this, sg_accept(S.f_out, coarse ? S.f_base : nullptr, S.f_texel, n_in, S.scale_p,
-                                 S.cfg.conf_threshold, d_mapping, S.f_pred, S.f_list, S.f_count, S.f_accepted, s));
-        LS_CHECK(this, hipMemcpyAsync(&n_fb, S.f_count, 4, hipMemcpyDeviceToHost, s));
-        LS_CHECK(this, hipStreamSynchronize(s));
+    if (coarse) {
+        ls::FlowmapParams Fc;
+        std::memset(&Fc, 0, sizeof(Fc));
+        Fc.step = surrogate_coarse_step_params();
+        Fc.states = S.f_states;
+        Fc.n = npx;
+        Fc.n_dev = n_in;
+        Fc.exits = S.f_base;
+        LS_CHECK(this, flowmap(Fc));
     }
-    LS_CHECK(this, hipEventRecord(e[2], s));
+    LS_CHECK(this, mark(6));
+    lm::MlpInferParams M;
+    std::memset(&M, 0, sizeof(M));
+    M.states = S.f_states;
+    M.n = npx;
+    M.n_dev = n_in;
+    M.radius = S.cfg.radius;
+    M.freqs = S.d_freqs;
+    M.wpack = S.d_wpack;
+    for (int l = 0; l < 4; ++l) M.bias[l] = S.b[l];
+    M.w_out = S.W[4];
+    M.b_out = S.b[4];
+    M.out = S.f_out;
+    LS_CHECK(this, sg_mlp_infer(M, s));
+    LS_CHECK(this, sg_accept(S.f_out, coarse ? S.f_base : nullptr, S.f_texel, npx, n_in, S.scale_p,
+                             S.cfg.conf_threshold, d_mapping, S.f_pred, S.f_list, n_fb, S.f_accepted, s));
+    LS_CHECK(this, mark(2));
     // 3. RK4 for the unconfident rays
     ls::FlowmapParams F;
     std::memset(&F, 0, sizeof(F));
     F.step = I;
     F.states = S.f_states;
     F.exits = S.f_exits;
-    if (n_fb > 0) {
-        F.index = S.f_list;
-        F.n = n_fb;
-        LS_CHECK(this, wgsl_math_for(S.cfg.field) ? linon_wgslmath::sg_flowmap(S.cfg.field, F, s)
-                                                    : sg_flowmap(S.cfg.field, F, s));
-        LS_CHECK(this, sg_apply_rk4(S.f_list, n_fb, S.f_exits, S.f_texel, S.f_pred, d_mapping, nullptr, 1, s));
-    }
-    LS_CHECK(this, hipEventRecord(e[3], s));
+    F.index = S.f_list;
+    F.n = npx;
+    F.n_dev = n_fb;
+    LS_CHECK(this, flowmap(F));
+    LS_CHECK(this, sg_apply_rk4(S.f_list, npx, n_fb, S.f_exits, S.f_texel, S.f_pred, d_mapping, nullptr, 1, s));
+    LS_CHECK(this, mark(3));
     // 4. Lyapunov gate (the overlay computes lambda from the mapping)
     OverlayParams O;
     std::memset(&O, 0, sizeof(O));
@@ -622,76 +629,71 @@ linon_status Context::surrogate_frame(hipStream_t s) {
     O.lyapunov = d_lyap;
     LS_CHECK(this, sg_fill_i32(reinterpret_cast<int*>(d_color), npx, (int)0xff000000u, s));  // black, alpha 1
     LS_CHECK(this, launch_overlay(O, s));
-    int n_ly = 0;
-    if (n_in > 0) {
-        LS_CHECK(this, sg_fill_i32(S.f_count, 1, 0, s));
-        LS_CHECK(this, sg_lyapunov_gate(S.f_accepted, S.f_texel, n_in, d_lyap, S.cfg.lyapunov_threshold, S.f_list,
-                                        S.f_count, s));
-        LS_CHECK(this, hipMemcpyAsync(&n_ly, S.f_count, 4, hipMemcpyDeviceToHost, s));
-        LS_CHECK(this, hipStreamSynchronize(s));
-        if (n_ly > 0) {
-            F.index = S.f_list;
-            F.n = n_ly;
-            LS_CHECK(this, wgsl_math_for(S.cfg.field) ? linon_wgslmath::sg_flowmap(S.cfg.field, F, s)
-                                                    : sg_flowmap(S.cfg.field, F, s));
-            LS_CHECK(this, sg_apply_rk4(S.f_list, n_ly, S.f_exits, S.f_texel, S.f_pred, d_mapping, nullptr, 1, s));
-            LS_CHECK(this, launch_overlay(O, s));
-        }
-    }
-    LS_CHECK(this, hipEventRecord(e[4], s));
-    LS_CHECK(this, sg_accepted_mask(S.f_accepted, S.f_texel, n_in, npx, S.f_mask, s));
+    LS_CHECK(this, sg_lyapunov_gate(S.f_accepted, S.f_texel, npx, n_in, d_lyap, S.cfg.lyapunov_threshold, S.f_list,
+                                    n_ly, s));
+    F.n_dev = n_ly;
+    LS_CHECK(this, flowmap(F));
+    LS_CHECK(this, sg_apply_rk4(S.f_list, npx, n_ly, S.f_exits, S.f_texel, S.f_pred, d_mapping, nullptr, 1, s));
+    LS_CHECK(this, launch_overlay(O, s));
+    LS_CHECK(this, mark(4));
+    LS_CHECK(this, sg_accepted_mask(S.f_accepted, S.f_texel, npx, n_in, npx, S.f_mask, s));
     // 5. re-judge a sample of the accepted rays
-    int n_rj = 0;
-    std::vector<float> rj;
-    if (n_in > 0 && S.cfg.rejudge_fraction > 0.0f) {
-        const int stride = std::max(1, (int)std::lround(1.0 / S.cfg.rejudge_fraction));
-        LS_CHECK(this, sg_fill_i32(S.f_count, 1, 0, s));
-        LS_CHECK(this, sg_pick_rejudge(S.f_accepted, n_in, stride, 0xC0FFEEull + S.frames, S.f_list, S.f_count, s));
-        LS_CHECK(this, hipMemcpyAsync(&n_rj, S.f_count, 4, hipMemcpyDeviceToHost, s));
-        LS_CHECK(this, hipStreamSynchronize(s));
-        if (n_rj > 0) {
-            F.index = S.f_list;
-            F.n = n_rj;
-            LS_CHECK(this, wgsl_math_for(S.cfg.field) ? linon_wgslmath::sg_flowmap(S.cfg.field, F, s)
-                                                    : sg_flowmap(S.cfg.field, F, s));
-            LS_CHECK(this, sg_apply_rk4(S.f_list, n_rj, S.f_exits, S.f_texel, S.f_pred, d_mapping, S.f_errs, 0, s));
-            rj.resize(n_rj);
-            LS_CHECK(this, hipMemcpyAsync(rj.data(), S.f_errs, (size_t)n_rj * 4, hipMemcpyDeviceToHost, s));
-        }
+    const int stride = S.cfg.rejudge_fraction > 0.0f ? std::max(1, (int)std::lround(1.0 / S.cfg.rejudge_fraction)) : 0;
+    if (stride > 0) {
+        LS_CHECK(this, sg_pick_rejudge(S.f_accepted, npx, n_in, stride, 0xC0FFEEull + S.frames, S.f_list, n_rj, s));
+        F.n_dev = n_rj;
+        LS_CHECK(this, flowmap(F));
+        LS_CHECK(this, sg_apply_rk4(S.f_list, npx, n_rj, S.f_exits, S.f_texel, S.f_pred, d_mapping, S.f_errs, 0, s));
     }
-    LS_CHECK(this, hipEventRecord(e[5], s));
-    LS_CHECK(this, hipEventSynchronize(e[5]));
+    LS_CHECK(this, mark(5));
+    S.frame_timed = !capturing;
+    S.stats_stale = true;
+    ++S.frames;
+    return LINON_OK;
+}
+
+// Counts, re-judged errors and (for an uncaptured frame) phase timings of the last frame.
+linon_status Context::surrogate_collect_stats() {
+    Surrogate& S = sg;
+    if (!S.stats_stale) return LINON_OK;
+    LS_CHECK(this, hipSetDevice(device));
+    LS_CHECK(this, hipDeviceSynchronize());
+    int cnt[4] = {0, 0, 0, 0};
+    LS_CHECK(this, hipMemcpy(cnt, S.f_count, sizeof(cnt), hipMemcpyDeviceToHost));
     linon_surrogate_stats& F2 = S.frame_stats;
     F2 = S.train_stats;
-    F2.frame_rays = (uint64_t)npx;
-    F2.entered = (uint64_t)n_in;
-    F2.conf_fallback = (uint64_t)n_fb;
-    F2.lyapunov_fallback = (uint64_t)n_ly;
-    F2.accepted = (uint64_t)(n_in - n_fb - n_ly);
-    F2.rejudged = (uint64_t)n_rj;
-    float ms[5];
-    for (int k = 0; k < 5; ++k) (void)hipEventElapsedTime(&ms[k], e[k], e[k + 1]);
-    F2.approach_ms = ms[0];
-    F2.coarse_ms = 0.0f;
-    F2.mlp_ms = ms[1];
-    if (n_in > 0) {  // the MLP phase: the coarse flow map, then the network
+    F2.frame_rays = (uint64_t)S.frame_px;
+    F2.entered = (uint64_t)cnt[0];
+    F2.conf_fallback = (uint64_t)cnt[1];
+    F2.lyapunov_fallback = (uint64_t)cnt[2];
+    F2.accepted = (uint64_t)(cnt[0] - cnt[1] - cnt[2]);
+    F2.rejudged = (uint64_t)cnt[3];
+    if (S.frame_timed) {
+        hipEvent_t* e = S.ev;
+        float ms[5];
+        for (int k = 0; k < 5; ++k) (void)hipEventElapsedTime(&ms[k], e[k], e[k + 1]);
+        F2.approach_ms = ms[0];
+        F2.mlp_ms = ms[1];  // coarse flow map + MLP + verdicts
+        F2.fallback_ms = ms[2];
+        F2.overlay_ms = ms[3];
+        F2.rejudge_ms = ms[4];
+        F2.frame_ms = ms[0] + ms[1] + ms[2] + ms[3];
         (void)hipEventElapsedTime(&F2.coarse_ms, e[1], e[6]);
-        (void)hipEventElapsedTime(&F2.mlp_ms, e[6], e[2]);
+        float mlp = 0.0f;
+        (void)hipEventElapsedTime(&mlp, e[6], e[2]);
+        F2.mlp_ms = mlp;
     }
-    F2.fallback_ms = ms[2];
-    F2.overlay_ms = ms[3];
-    F2.rejudge_ms = ms[4];
-    F2.frame_ms = ms[0] + ms[1] + ms[2] + ms[3];
-    if (!rj.empty()) {
+    if (cnt[3] > 0) {
+        std::vector<float> rj(cnt[3]);
+        LS_CHECK(this, hipMemcpy(rj.data(), S.f_errs, (size_t)cnt[3] * 4, hipMemcpyDeviceToHost));
         std::sort(rj.begin(), rj.end());
         F2.rejudge_err_q50 = rj[rj.size() / 2];
         F2.rejudge_err_q99 = rj[std::min(rj.size() - 1, (size_t)(0.99 * rj.size()))];
         F2.rejudge_err_max = rj.back();
         F2.rejudge_frac_within_tol =
-            (float)(std::lower_bound(rj.begin(), rj.end(), S.cfg.tolerance) - rj.begin()) / (float)rj.size();
+            (float)(std::upper_bound(rj.begin(), rj.end(), S.cfg.tolerance) - rj.begin()) / (float)rj.size();
     }
-    ++S.frames;
-    destroy();
+    S.stats_stale = false;
     return LINON_OK;
 }
 
@@ -863,6 +865,11 @@ linon_status linon_surrogate_infer(linon_context* ctx, const float* states, int 
 linon_status linon_get_surrogate_stats(linon_context* ctx, linon_surrogate_stats* out) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c || !out) return LINON_ERR_ARGUMENT;
+    c->err.clear();
+    if (c->sg.frames) {
+        const linon_status st = c->surrogate_collect_stats();
+        if (st != LINON_OK) return st;
+    }
     *out = c->sg.frames ? c->sg.frame_stats : c->sg.train_stats;
     return LINON_OK;
 }

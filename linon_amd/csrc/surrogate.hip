@@ -64,10 +64,10 @@ __global__ void __launch_bounds__(256) sample_states_kernel(unsigned long long s
 }
 
 // Picks roughly one accepted ray in `stride` (hash-spread) for re-judging with RK4.
-__global__ void __launch_bounds__(256) pick_rejudge_kernel(const int* accepted, int n, int stride,
+__global__ void __launch_bounds__(256) pick_rejudge_kernel(const int* accepted, int n, const int* n_dev, int stride,
                                                            unsigned long long seed, int* list, int* count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !accepted[i]) return;
+    if (i >= (n_dev ? *n_dev : n) || !accepted[i]) return;
     if ((int)(u01(seed, (unsigned long long)i) * (float)stride) != 0) return;
     int slot = atomicAdd(count, 1);
     list[slot] = i;
@@ -279,11 +279,11 @@ __global__ void __launch_bounds__(256) adam_kernel(float* w, float* m, float* v,
 // confidence reaches the threshold; otherwise queue the ray for RK4.  The exit point is the
 // base (coarse flow map, or 0) plus the output over the point scale.
 __global__ void __launch_bounds__(256) accept_kernel(const float* out, const float* base, const int* texel_of, int n,
-                                                     float scale_p, float conf_threshold, float* mapping,
-                                                     float* pred_exit, int* fallback, int* n_fallback,
-                                                     int* accepted) {
+                                                     const int* n_dev, float scale_p, float conf_threshold,
+                                                     float* mapping, float* pred_exit, int* fallback,
+                                                     int* n_fallback, int* accepted) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= (n_dev ? *n_dev : n)) return;
     const float* o = out + (size_t)i * LS_OUT;
     const float conf = 1.0f / (1.0f + __expf(-o[6]));
     const int texel = texel_of[i];
@@ -326,11 +326,11 @@ __global__ void __launch_bounds__(256) compact_entered_kernel(const int* entered
 
 // Rays re-judged by RK4 or sent to RK4: write the RK4 exit into the mapping; for re-judged
 // rays also record the surrogate's error against the RK4 exit (exit_error).
-__global__ void __launch_bounds__(256) apply_rk4_kernel(const int* list, int n, const float* exits,
+__global__ void __launch_bounds__(256) apply_rk4_kernel(const int* list, int n, const int* n_dev, const float* exits,
                                                         const int* texel_of, const float* pred_exit, float* mapping,
                                                         float* err_out, int write_mapping) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= (n_dev ? *n_dev : n)) return;
     const int r = list[i];
     const float* e = exits + 6 * (size_t)r;
     if (err_out)
@@ -341,10 +341,10 @@ __global__ void __launch_bounds__(256) apply_rk4_kernel(const int* list, int n, 
 // Accepted rays whose Lyapunov exponent (from the surrogate's own mapping) is above the
 // threshold are queued for RK4 as well, and no longer count as accepted.
 __global__ void __launch_bounds__(256) lyapunov_gate_kernel(int* accepted, const int* texel_of, int n,
-                                                            const float* lyap, float threshold, int* list,
-                                                            int* count) {
+                                                            const int* n_dev, const float* lyap, float threshold,
+                                                            int* list, int* count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !accepted[i]) return;
+    if (i >= (n_dev ? *n_dev : n) || !accepted[i]) return;
     const float l = lyap[texel_of[i]];
     if (l > threshold) {
         accepted[i] = 0;
@@ -355,9 +355,9 @@ __global__ void __launch_bounds__(256) lyapunov_gate_kernel(int* accepted, const
 
 // Per texel: 1 where the frame kept the surrogate's exit (mask cleared beforehand).
 __global__ void __launch_bounds__(256) accepted_mask_kernel(const int* accepted, const int* texel_of, int n,
-                                                            unsigned char* mask) {
+                                                            const int* n_dev, unsigned char* mask) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && accepted[i]) mask[texel_of[i]] = 1;
+    if (i < (n_dev ? *n_dev : n) && accepted[i]) mask[texel_of[i]] = 1;
 }
 
 __global__ void __launch_bounds__(256) fill_u8_kernel(unsigned char* p, int n, unsigned char value) {
@@ -442,10 +442,11 @@ hipError_t sg_adam(float* w, float* m, float* v, const float* g, int K, int N, f
     hipLaunchKernelGGL(adam_kernel, LS_GRID(K * N), 0, s, w, m, v, g, K, N, lr, b1, b2, eps, bc1, bc2, wt, ldt);
     return hipGetLastError();
 }
-hipError_t sg_accept(const float* out, const float* base, const int* texel_of, int n, float scale_p, float thr,
-                     float* mapping, float* pred_exit, int* fallback, int* n_fallback, int* accepted, hipStream_t s) {
-    hipLaunchKernelGGL(accept_kernel, LS_GRID(n), 0, s, out, base, texel_of, n, scale_p, thr, mapping, pred_exit,
-                       fallback, n_fallback, accepted);
+hipError_t sg_accept(const float* out, const float* base, const int* texel_of, int n, const int* n_dev, float scale_p,
+                     float thr, float* mapping, float* pred_exit, int* fallback, int* n_fallback, int* accepted,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(accept_kernel, LS_GRID(n), 0, s, out, base, texel_of, n, n_dev, scale_p, thr, mapping,
+                       pred_exit, fallback, n_fallback, accepted);
     return hipGetLastError();
 }
 hipError_t sg_compact_entered(const int* entered, const float* entry_states, int n, float* states, int* texel_of,
@@ -453,30 +454,30 @@ hipError_t sg_compact_entered(const int* entered, const float* entry_states, int
     hipLaunchKernelGGL(compact_entered_kernel, LS_GRID(n), 0, s, entered, entry_states, n, states, texel_of, count);
     return hipGetLastError();
 }
-hipError_t sg_apply_rk4(const int* list, int n, const float* exits, const int* texel_of, const float* pred_exit,
-                        float* mapping, float* err_out, int write_mapping, hipStream_t s) {
-    hipLaunchKernelGGL(apply_rk4_kernel, LS_GRID(n), 0, s, list, n, exits, texel_of, pred_exit, mapping, err_out,
+hipError_t sg_apply_rk4(const int* list, int n, const int* n_dev, const float* exits, const int* texel_of,
+                        const float* pred_exit, float* mapping, float* err_out, int write_mapping, hipStream_t s) {
+    hipLaunchKernelGGL(apply_rk4_kernel, LS_GRID(n), 0, s, list, n, n_dev, exits, texel_of, pred_exit, mapping, err_out,
                        write_mapping);
     return hipGetLastError();
 }
-hipError_t sg_accepted_mask(const int* accepted, const int* texel_of, int n, int npx, unsigned char* mask,
-                            hipStream_t s) {
+hipError_t sg_accepted_mask(const int* accepted, const int* texel_of, int n, const int* n_dev, int npx,
+                            unsigned char* mask, hipStream_t s) {
     hipLaunchKernelGGL(fill_u8_kernel, LS_GRID(npx), 0, s, mask, npx, (unsigned char)0);
-    if (n > 0) hipLaunchKernelGGL(accepted_mask_kernel, LS_GRID(n), 0, s, accepted, texel_of, n, mask);
+    if (n > 0) hipLaunchKernelGGL(accepted_mask_kernel, LS_GRID(n), 0, s, accepted, texel_of, n, n_dev, mask);
     return hipGetLastError();
 }
-hipError_t sg_lyapunov_gate(int* accepted, const int* texel_of, int n, const float* lyap, float thr, int* list,
-                            int* count, hipStream_t s) {
-    hipLaunchKernelGGL(lyapunov_gate_kernel, LS_GRID(n), 0, s, accepted, texel_of, n, lyap, thr, list, count);
+hipError_t sg_lyapunov_gate(int* accepted, const int* texel_of, int n, const int* n_dev, const float* lyap, float thr,
+                            int* list, int* count, hipStream_t s) {
+    hipLaunchKernelGGL(lyapunov_gate_kernel, LS_GRID(n), 0, s, accepted, texel_of, n, n_dev, lyap, thr, list, count);
     return hipGetLastError();
 }
 hipError_t sg_sample_states(unsigned long long seed, int n, float radius, float inset, float* states, hipStream_t s) {
     hipLaunchKernelGGL(sample_states_kernel, LS_GRID(n), 0, s, seed, n, radius, inset, states);
     return hipGetLastError();
 }
-hipError_t sg_pick_rejudge(const int* accepted, int n, int stride, unsigned long long seed, int* list, int* count,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(pick_rejudge_kernel, LS_GRID(n), 0, s, accepted, n, stride, seed, list, count);
+hipError_t sg_pick_rejudge(const int* accepted, int n, const int* n_dev, int stride, unsigned long long seed, int* list,
+                           int* count, hipStream_t s) {
+    hipLaunchKernelGGL(pick_rejudge_kernel, LS_GRID(n), 0, s, accepted, n, n_dev, stride, seed, list, count);
     return hipGetLastError();
 }
 hipError_t sg_fill_i32(int* p, int n, int value, hipStream_t s) {

@@ -33,21 +33,21 @@
     hipError_t sg_colsum(const lm::u16* dz, int n, int cols, float* out, hipStream_t s); \
     hipError_t sg_adam(float* w, float* m, float* v, const float* g, int K, int N, float lr, float b1, float b2, float eps, \
                        float bc1, float bc2, lm::u16* wt, int ldt, hipStream_t s); \
-    hipError_t sg_accept(const float* out, const float* base, const int* texel_of, int n, float scale_p, float thr, \
-                         float* mapping, float* pred_exit, int* fallback, int* n_fallback, int* accepted, \
-                         hipStream_t s); \
+    hipError_t sg_accept(const float* out, const float* base, const int* texel_of, int n, const int* n_dev, \
+                         float scale_p, float thr, float* mapping, float* pred_exit, int* fallback, int* n_fallback, \
+                         int* accepted, hipStream_t s); \
     hipError_t sg_compact_entered(const int* entered, const float* entry_states, int n, float* states, int* texel_of, \
                                   int* count, hipStream_t s); \
-    hipError_t sg_apply_rk4(const int* list, int n, const float* exits, const int* texel_of, const float* pred_exit, \
-                            float* mapping, float* err_out, int write_mapping, hipStream_t s); \
-    hipError_t sg_accepted_mask(const int* accepted, const int* texel_of, int n, int npx, unsigned char* mask, \
-                                hipStream_t s); \
-    hipError_t sg_lyapunov_gate(int* accepted, const int* texel_of, int n, const float* lyap, float thr, int* list, \
-                                int* count, hipStream_t s); \
+    hipError_t sg_apply_rk4(const int* list, int n, const int* n_dev, const float* exits, const int* texel_of, \
+                            const float* pred_exit, float* mapping, float* err_out, int write_mapping, hipStream_t s); \
+    hipError_t sg_accepted_mask(const int* accepted, const int* texel_of, int n, const int* n_dev, int npx, \
+                                unsigned char* mask, hipStream_t s); \
+    hipError_t sg_lyapunov_gate(int* accepted, const int* texel_of, int n, const int* n_dev, const float* lyap, \
+                                float thr, int* list, int* count, hipStream_t s); \
     hipError_t sg_fill_i32(int* p, int n, int value, hipStream_t s); \
     hipError_t sg_sample_states(unsigned long long seed, int n, float radius, float inset, float* states, hipStream_t s); \
-    hipError_t sg_pick_rejudge(const int* accepted, int n, int stride, unsigned long long seed, int* list, int* count, \
-                               hipStream_t s); \
+    hipError_t sg_pick_rejudge(const int* accepted, int n, const int* n_dev, int stride, unsigned long long seed, \
+                               int* list, int* count, hipStream_t s); \
     hipError_t sg_approach(int field, const ls::ApproachParams& A, hipStream_t s); \
     hipError_t sg_flowmap(int field, const ls::FlowmapParams& P, hipStream_t s);
 

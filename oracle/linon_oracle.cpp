@@ -215,12 +215,17 @@ struct Scene {
     uint32_t nf = 0;
     // per-face bounding boxes, widened by `pad`, for the optional exact prefilter of ray_color
     std::vector<T> box;  // 6 per face
+    // the union of the boxes of each run of kPrefilterBlock consecutive faces (6 per block)
+    std::vector<T> block_box;
     T pad = T(0);
 };
+constexpr uint32_t kPrefilterBlock = 256;
 
 // Prefilter switch: skip the Moeller-Trumbore test of a face whose (slightly widened) bounding
-// box misses the segment's bounding box.  Such a face cannot produce 0 < t < max_dist, so the
-// result is unchanged; it only saves time on large meshes.  tests/ checks on == off.
+// box misses the segment's bounding box, and a whole run of consecutive faces whose box union
+// misses it.  Such a face cannot produce 0 < t < max_dist, and the faces left are still visited
+// in ascending order, so the result is unchanged; it only saves time on large meshes.  tests/
+// checks on == off.
 bool g_prefilter = true;
 
 template <typename T>
@@ -263,6 +268,14 @@ Hit<T> ray_color(const Scene<T>& sc, V3<T> origin, V3<T> direction, T max_dist, 
     const T shi[3] = {std::max(origin.x, end.x), std::max(origin.y, end.y), std::max(origin.z, end.z)};
     long long tested = 0;
     for (uint32_t i = 0; i < sc.nf; ++i) {
+        if (pre && i % kPrefilterBlock == 0) {
+            const T* bb = &sc.block_box[6 * (size_t)(i / kPrefilterBlock)];
+            if (bb[0] > shi[0] || bb[3] < slo[0] || bb[1] > shi[1] || bb[4] < slo[1] || bb[2] > shi[2] ||
+                bb[5] < slo[2]) {
+                i += kPrefilterBlock - 1;  // the loop's ++i moves to the next block
+                continue;
+            }
+        }
         if (pre) {
             const T* bx = &sc.box[6 * (size_t)i];
             if (bx[0] > shi[0] || bx[3] < slo[0] || bx[1] > shi[1] || bx[4] < slo[1] || bx[2] > shi[2] ||
@@ -507,6 +520,15 @@ Scene<T> make_scene(const float* verts, uint32_t nv, const uint32_t* faces, uint
             sc.box[6 * (size_t)i + k] -= sc.pad;
             sc.box[6 * (size_t)i + 3 + k] += sc.pad;
         }
+    sc.block_box.assign(6 * (size_t)((nf + kPrefilterBlock - 1) / kPrefilterBlock), T(0));
+    for (uint32_t i = 0; i < nf; ++i) {
+        T* bb = &sc.block_box[6 * (size_t)(i / kPrefilterBlock)];
+        const T* bx = &sc.box[6 * (size_t)i];
+        for (int k = 0; k < 3; ++k) {
+            bb[k] = i % kPrefilterBlock == 0 ? bx[k] : std::min(bb[k], bx[k]);
+            bb[3 + k] = i % kPrefilterBlock == 0 ? bx[3 + k] : std::max(bb[3 + k], bx[3 + k]);
+        }
+    }
     return sc;
 }
 

@@ -16,7 +16,10 @@ def _grid(view, builder):
     return view.read_grid()
 
 
-@pytest.mark.parametrize("mesh,res", [("cornell", 0), ("ico3", 0), ("ico6", 128), ("ico6", 256), ("empty", 0)])
+# ico8 is the 4K stress scene: the Cornell box plus the level-8 icosphere (1,310,756 faces) at
+# the resolution the context picks for it
+@pytest.mark.parametrize("mesh,res", [("cornell", 0), ("ico3", 0), ("ico6", 128), ("ico6", 256), ("ico8", 0),
+                                      ("empty", 0)])
 def test_device_grid_equals_host_grid(mesh, res):
     if mesh == "cornell":
         v, f = scenes.cornell_box_centered()

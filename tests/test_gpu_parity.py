@@ -415,3 +415,41 @@ def test_outline_search_at_4k(field):
     assert (got[:, 2].sum() > 0) == (n_hi > 0)
     if field == "lens":
         assert n_hi > 0
+
+
+@pytest.mark.parametrize("field", ["lens", "hot_air"])
+def test_4k_stress_frame_sample(field):
+    """The 3840x2160 stress frame (Cornell box + the 1,310,720-face level-8 icosphere, h 0.001,
+    5001 RK4 steps per ray, overlay, outline rays): the whole frame on the GPU, 256 random pixels
+    through the FP32 and FP64 oracles (brute force over all faces, with the exact block
+    prefilter); lambda from the GPU's own overlay, scaled to the 1080-row pitch."""
+    wl = configs.STRESS_4K
+    v, f = wl.mesh()
+    with L.MainView(wl.width, wl.height) as big:
+        big.update_model(v, f)
+        configs.setup_view(big, wl, field)
+        print(f"[case] 4K stress frame {field}", flush=True)
+        big.render()
+        fr = big.read_frame()
+    rng = np.random.default_rng(11)
+    gy = rng.integers(0, wl.height, 256)
+    gx = rng.integers(0, wl.width, 256)
+    px = np.stack([gx, gy], 1)
+    ty = wl.height - 1 - gy
+    r32, ref64 = oracle_pixels(wl, L.FIELDS[field], v, f, px)
+
+    class G:
+        pass
+    g = G()
+    g.hit_faces = fr.hit_faces[ty, gx][None]
+    g.mapping = fr.mapping[ty, gx][None]
+    g.ray_casting = fr.ray_casting[ty, gx][None]
+    ref = {"mapping": r32["mapping"][None], "hit": r32["hit"][None], "noise_flag": np.zeros((1, len(px)), np.uint8),
+           "color8": g.ray_casting}
+    lam = fr.lyapunov[ty, gx][None] * (wl.height / 1080)
+    info = compare(g, ref, ref64, f"4k-{field}", lam_gpu=lam)
+    assert info["gated"] > 30, info
+    assert info["hit_ok"], info
+    assert info["exit_pass"], info
+    assert info["q99_ok"], info
+    assert (r32["hit"] >= 0).any()
