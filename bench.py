@@ -222,7 +222,8 @@ def main() -> int:
     ap.add_argument("--surrogate-steps", type=int, default=configs.SURROGATE_STEPS)
     ap.add_argument("--surrogate-train", action="store_true",
                     help="train the surrogates in this run even where saved weights match")
-    ap.add_argument("--surrogate-fields", default="swirl,lens,hot_air,warp")
+    ap.add_argument("--surrogate-fields", default="swirl,lens,hot_air,warp",
+                    help="fields of the 1080p surrogate leg (empty: none; the stress leg keeps its own)")
     ap.add_argument("--skip-stress", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device (the reference transfer that scales the "
@@ -338,15 +339,11 @@ def main() -> int:
         extra["batched_views"] = bench_batch(verts, faces, a.batch_replays)
     if not a.skip_surrogate:
         log("surrogate ...")
-        extra["surrogate"] = bench_surrogate(a.surrogate_steps, a.surrogate_fields.split(","),
+        extra["surrogate"] = bench_surrogate(a.surrogate_steps, [f for f in a.surrogate_fields.split(",") if f],
                                              train=a.surrogate_train)
     if not a.skip_stress:
         log("4K stress ...")
-        extra["stress_4k"] = bench_stress()
-        if not a.skip_surrogate:
-            log("4K stress surrogate ...")
-            extra["stress_4k"]["surrogate"] = bench_surrogate(a.surrogate_steps, configs.STRESS_4K.fields,
-                                                              configs.STRESS_4K, train=a.surrogate_train)
+        extra["stress_4k"] = bench_stress(0 if a.skip_surrogate else a.surrogate_steps, a.surrogate_train)
     out["other_workloads"] = extra
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
@@ -359,10 +356,10 @@ def main() -> int:
     return 0
 
 
-def one_graph_ms(view, fields) -> float:
-    """Captures one render per field on a new stream into a single HIP graph (through the HIP
-    runtime the library already runs on), replays it once to warm up and returns the device time
-    of one more replay."""
+def one_graph_ms(frames) -> float:
+    """Captures `frames` (callables that enqueue one frame on the stream they are given) in order
+    on a new stream into a single HIP graph (through the HIP runtime the library already runs
+    on), replays it once to warm up and returns the device time of one more replay."""
     hip = C.CDLL("libamdhip64.so.7")
     vp = C.c_void_p
 
@@ -375,9 +372,8 @@ def one_graph_ms(view, fields) -> float:
     ok(hip.hipEventCreate(C.byref(e0)), "hipEventCreate")
     ok(hip.hipEventCreate(C.byref(e1)), "hipEventCreate")
     ok(hip.hipStreamBeginCapture(stream, 1), "hipStreamBeginCapture")  # thread-local mode
-    for name in fields:
-        view.set_field(name)
-        view.render(stream=stream.value)
+    for frame in frames:
+        frame(stream.value)
     ok(hip.hipStreamEndCapture(stream, C.byref(graph)), "hipStreamEndCapture")
     ok(hip.hipGraphInstantiateWithFlags(C.byref(exe), graph, C.c_ulonglong(0)), "hipGraphInstantiateWithFlags")
     ok(hip.hipGraphLaunch(exe, stream), "hipGraphLaunch")
@@ -395,10 +391,19 @@ def one_graph_ms(view, fields) -> float:
     return float(ms.value)
 
 
-def bench_stress() -> dict:
+def plain_frame(view, field):
+    def enqueue(stream):
+        view.set_field(field)
+        view.render(stream=stream)
+    return enqueue
+
+
+def bench_stress(surrogate_steps: int = 0, train: bool = False) -> dict:
     """3840x2160 at h = 0.001 (5001 RK4 iterations per ray) against the Cornell box plus the
     1,310,720-face level-8 icosphere, lens and hot-air, overlay on, 8 outline rays x 64 records:
-    one warm-up and one timed frame per field, then both frames in one captured HIP graph."""
+    one warm-up and one timed frame per field.  With surrogate_steps > 0, also the surrogate +
+    fallback frame of each field at the same size and step (surrogate_leg), and then every frame
+    -- the plain frames of both fields and their surrogate frames -- in one captured HIP graph."""
     wl = configs.STRESS_4K
     verts, faces = wl.mesh()
     view = L.MainView(wl.width, wl.height)
@@ -433,10 +438,26 @@ def bench_stress() -> dict:
                                                              verts, faces, wl.width, wl.height, fr.mapping,
                                                              fr.hit_faces, 64, seed=2)
         log(f"stress {name}: {rms[0]:.1f} ms")
-    # both fields' frames captured into one HIP graph on a caller's stream and replayed (the
-    # graph keeps each capture's field; the frames share the output buffers)
-    out["one_graph_all_fields_ms"] = one_graph_ms(view, wl.fields)
+    frames = [plain_frame(view, name) for name in wl.fields]
+    # the plain frames of both fields captured into one HIP graph on a caller's stream and
+    # replayed (the graph keeps each capture's field; the frames share the output buffers)
+    out["one_graph_all_fields_ms"] = one_graph_ms(frames)
     log(f"stress one graph ({'+'.join(wl.fields)}): {out['one_graph_all_fields_ms']:.1f} ms")
+    if surrogate_steps > 0:
+        sviews = []
+        out["surrogate"] = {}
+        for name in wl.fields:
+            sv, res = surrogate_leg(name, surrogate_steps, wl, train)
+            out["surrogate"][name] = res
+            sviews.append(sv)
+            frames.append(lambda stream, sv=sv: sv.render_surrogate(stream=stream))
+        # every frame of the stress configuration in one graph: plain lens and hot air with the
+        # mesh, overlay and outline rays, then the surrogate + fallback frames of both fields
+        out["one_graph_plain_and_surrogate_ms"] = one_graph_ms(frames)
+        out["one_graph_frames"] = [f"plain {n}" for n in wl.fields] + [f"surrogate {n}" for n in wl.fields]
+        log(f"stress one graph (plain + surrogate frames): {out['one_graph_plain_and_surrogate_ms']:.1f} ms")
+        for sv in sviews:
+            sv.close()
     view.close()
     return out
 
@@ -502,70 +523,74 @@ def bench_batch(verts, faces, replays: int) -> dict:
             "accuracy": accuracy}
 
 
+def surrogate_leg(field: str, steps: int, wl=configs.OVERLAY_1080P, train: bool = False):
+    """The flow-map surrogate of the workload's step h for one field (configs.surrogate_config;
+    the saved weights under linon_amd/surrogates/ when they match it and `train` is off, else
+    trained here on 2^24 uniform entry states labelled by the FP32 RK4 kernel), then the
+    workload's view without the mesh (mapping + overlay), rendered twice: plain RK4 (escape mode,
+    exits on the sphere) and surrogate + fallback.  Returns the view (holding the surrogate) and
+    the speed-up and, over the texels that took the surrogate's exit, the fraction within the
+    1e-3 tolerance of the plain frame."""
+    view = L.MainView(wl.width, wl.height)
+    view.update_model(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
+    view.update_camera(wl.camera())
+    view.update_settings(wl.settings())
+    cfg = configs.surrogate_config(field, wl, steps=steps)
+    saved = configs.saved_surrogate(field, wl)
+    if saved is not None and not train:
+        view.load_surrogate(saved)
+        tr = dict(view.surrogate_stats(), source=f"saved weights ({saved.name})")
+    else:
+        tr = dict(view.train_surrogate(cfg), source="trained in this run")
+    ig = L.Integrator.make(h_initial=wl.h, t_max=1e30, adaptive=False, max_steps=wl.max_steps,
+                           mode=L.MODE_ESCAPE, escape_radius=2.0, samples_per_ray=0, sample_steps=0,
+                           need_mapping=True, clip_exit=True)
+    view.set_integrator(ig)
+    view.set_field(field)
+    view.render()
+    view.reset_timings()
+    view.render()
+    plain_ms = float(view.timings()[1][0])
+    plain = view.read_mapping()
+    view.render_surrogate()            # warm
+    view.render_surrogate()
+    st = view.surrogate_stats()
+    sur = view.read_mapping()
+    acc = view.read_surrogate_accepted()
+    a = sur[..., :3].astype(np.float64)[acc]
+    b = plain[..., :3].astype(np.float64)[acc]
+    rel = (np.abs(a - b) / np.maximum(1.0, np.abs(b))).max(axis=-1) if acc.any() else np.zeros(0)
+    res = {"train": {k: tr.get(k) for k in ("source", "train_loss", "label_seconds", "train_seconds",
+                                            "holdout_err_q50", "holdout_err_q99", "holdout_frac_within_tol")},
+           "h": wl.h, "max_steps": wl.max_steps, "coarse_h": cfg.coarse_h, "tolerance": cfg.tolerance,
+           "plain_frame_ms": plain_ms, "surrogate_frame_ms": st["frame_ms"],
+           "speedup": plain_ms / st["frame_ms"] if st["frame_ms"] > 0 else None,
+           # exit-point error (max over x, y, z of |d| / max(1, |plain|)) of the texels that took
+           # the surrogate's exit, against the plain frame
+           "accepted_within_tol": float((rel <= cfg.tolerance).mean()) if acc.any() else None,
+           "accepted_err_q50_q99_max": ([float(np.quantile(rel, q)) for q in (0.5, 0.99, 1.0)]
+                                        if acc.any() else None),
+           "nothing_accepted": not bool(acc.any()),
+           "rays": int(st["frame_rays"]), "entered": int(st["entered"]), "accepted": int(st["accepted"]),
+           "accepted_frac_of_entered": (int(st["accepted"]) / int(st["entered"]) if st["entered"] else None),
+           "conf_fallback": int(st["conf_fallback"]), "lyapunov_fallback": int(st["lyapunov_fallback"]),
+           "rejudged": int(st["rejudged"]), "rejudge_err_q50": st["rejudge_err_q50"],
+           "rejudge_err_q99": st["rejudge_err_q99"], "rejudge_err_max": st["rejudge_err_max"],
+           "phases_ms": {k: st[k] for k in ("approach_ms", "coarse_ms", "mlp_ms", "fallback_ms", "overlay_ms")},
+           # algorithmic MLP FLOP per ray: 2 x (198*256 + 3*256*256 + 256*7) = 498,176
+           "mlp_tflops": (int(st["entered"]) * 498176 / (st["mlp_ms"] * 1e-3) / 1e12
+                          if st["mlp_ms"] > 0 else None)}
+    log(f"surrogate {field} {wl.width}x{wl.height}: {res['speedup']}x, accepted "
+        f"{res['accepted']}/{res['entered']}, within tol {res['accepted_within_tol']}")
+    return view, res
+
+
 def bench_surrogate(steps: int, fields, wl=configs.OVERLAY_1080P, train: bool = False) -> dict:
-    """Per field: the flow-map surrogate of the workload's step h (configs.surrogate_config; the
-    saved weights under linon_amd/surrogates/ when they match it and `train` is off, else trained
-    here on 2^24 uniform entry states labelled by the FP32 RK4 kernel), then the workload's view
-    without the mesh (mapping + overlay), rendered twice: plain RK4 (escape mode, exits on the
-    sphere) and surrogate + fallback.  Reports the speed-up and, over the texels that took the
-    surrogate's exit, the fraction within the 1e-3 tolerance of the plain frame."""
+    """surrogate_leg for each field."""
     out = {}
     for field in fields:
-        view = L.MainView(wl.width, wl.height)
-        view.update_model(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
-        view.update_camera(wl.camera())
-        view.update_settings(wl.settings())
-        cfg = configs.surrogate_config(field, wl, steps=steps)
-        saved = configs.saved_surrogate(field, wl)
-        if saved is not None and not train:
-            view.load_surrogate(saved)
-            tr = dict(view.surrogate_stats(), source=f"saved weights ({saved.name})")
-        else:
-            tr = dict(view.train_surrogate(cfg), source="trained in this run")
-        ig = L.Integrator.make(h_initial=wl.h, t_max=1e30, adaptive=False, max_steps=wl.max_steps,
-                               mode=L.MODE_ESCAPE, escape_radius=2.0, samples_per_ray=0, sample_steps=0,
-                               need_mapping=True, clip_exit=True)
-        view.set_integrator(ig)
-        view.set_field(field)
-        view.render()
-        view.reset_timings()
-        view.render()
-        plain_ms = float(view.timings()[1][0])
-        plain = view.read_mapping()
-        view.render_surrogate()            # warm
-        view.render_surrogate()
-        st = view.surrogate_stats()
-        sur = view.read_mapping()
-        acc = view.read_surrogate_accepted()
-        a = sur[..., :3].astype(np.float64)[acc]
-        b = plain[..., :3].astype(np.float64)[acc]
-        rel = (np.abs(a - b) / np.maximum(1.0, np.abs(b))).max(axis=-1) if acc.any() else np.zeros(0)
-        out[field] = {"train": {k: tr.get(k) for k in ("source", "train_loss", "label_seconds", "train_seconds",
-                                                      "holdout_err_q50", "holdout_err_q99",
-                                                      "holdout_frac_within_tol")},
-                      "coarse_h": cfg.coarse_h, "tolerance": cfg.tolerance,
-                      "plain_frame_ms": plain_ms, "surrogate_frame_ms": st["frame_ms"],
-                      "speedup": plain_ms / st["frame_ms"] if st["frame_ms"] > 0 else None,
-                      # exit-point error (max over x, y, z of |d| / max(1, |plain|)) of the texels
-                      # that took the surrogate's exit, against the plain frame
-                      "accepted_within_tol": float((rel <= cfg.tolerance).mean()) if acc.any() else None,
-                      "accepted_err_q50_q99_max": ([float(np.quantile(rel, q)) for q in (0.5, 0.99, 1.0)]
-                                                   if acc.any() else None),
-                      "nothing_accepted": not bool(acc.any()),
-                      "rays": int(st["frame_rays"]), "entered": int(st["entered"]), "accepted": int(st["accepted"]),
-                      "accepted_frac_of_entered": (int(st["accepted"]) / int(st["entered"])
-                                                   if st["entered"] else None),
-                      "conf_fallback": int(st["conf_fallback"]), "lyapunov_fallback": int(st["lyapunov_fallback"]),
-                      "rejudged": int(st["rejudged"]), "rejudge_err_q50": st["rejudge_err_q50"],
-                      "rejudge_err_q99": st["rejudge_err_q99"], "rejudge_err_max": st["rejudge_err_max"],
-                      "phases_ms": {k: st[k] for k in ("approach_ms", "coarse_ms", "mlp_ms", "fallback_ms",
-                                                       "overlay_ms")},
-                      # algorithmic MLP FLOP per ray: 2 x (198*256 + 3*256*256 + 256*7) = 498,176
-                      "mlp_tflops": (int(st["entered"]) * 498176 / (st["mlp_ms"] * 1e-3) / 1e12
-                                     if st["mlp_ms"] > 0 else None)}
+        view, out[field] = surrogate_leg(field, steps, wl, train)
         view.close()
-        log(f"surrogate {field} {wl.width}x{wl.height}: {out[field]['speedup']}x, accepted "
-            f"{out[field]['accepted']}/{out[field]['entered']}, within tol {out[field]['accepted_within_tol']}")
     return out
 
 

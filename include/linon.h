@@ -317,7 +317,12 @@ linon_status linon_autotune_batch(linon_context* ctx, int downscale, linon_batch
  * view without the mesh (mapping texture + Lyapunov overlay; the ray-cast colour is black):
  * RK4 to the sphere, MLP for the rays that entered, RK4 for unconfident and high-lambda rays,
  * and RK4 on a sample of the accepted rays to measure the surrogate's error.  The comparable
- * plain frame is linon_render with an escape-mode integrator (need_mapping) and an empty mesh. */
+ * plain frame is linon_render with an escape-mode integrator (need_mapping) and an empty mesh.
+ * The frame never waits for the host (its compacted counts stay on the device), so a caller
+ * may capture it on `stream` into a HIP graph, with other views' frames, once one frame at this
+ * size has run uncaptured.  linon_get_surrogate_stats reads the counts and re-judged errors of
+ * the last frame that ran on the device (eager or replayed) and the phase timings of the last
+ * frame enqueued outside a capture (zero when the last one was captured). */
 linon_status linon_surrogate_train(linon_context* ctx, const linon_surrogate_config* cfg,
                                    linon_surrogate_stats* stats);
 linon_status linon_render_surrogate(linon_context* ctx, void* stream);

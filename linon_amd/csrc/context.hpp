@@ -84,8 +84,8 @@ struct Surrogate {
     linon_surrogate_stats train_stats{};
     linon_surrogate_stats frame_stats{};
     hipEvent_t ev[7] = {};      // frame phase boundaries
-    bool frame_timed = false;   // the last frame recorded them (it was not captured)
-    bool stats_stale = false;   // frame_stats not yet read back for the last frame
+    bool frame_timed = false;   // the last frame run from the host recorded them (it was not captured)
+    bool has_frame = false;     // a frame has been rendered or captured
     void release();
 };
 

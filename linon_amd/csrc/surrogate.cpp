@@ -647,15 +647,17 @@ linon_status Context::surrogate_frame(hipStream_t s) {
     }
     LS_CHECK(this, mark(5));
     S.frame_timed = !capturing;
-    S.stats_stale = true;
+    S.has_frame = true;
     ++S.frames;
     return LINON_OK;
 }
 
-// Counts, re-judged errors and (for an uncaptured frame) phase timings of the last frame.
+// Counts and re-judged errors of the last frame that ran on the device (read back on every
+// call: a replayed graph rewrites them without the host), and the phase timings when the last
+// frame the host enqueued was not captured (zero otherwise).
 linon_status Context::surrogate_collect_stats() {
     Surrogate& S = sg;
-    if (!S.stats_stale) return LINON_OK;
+    if (!S.has_frame) return LINON_OK;
     LS_CHECK(this, hipSetDevice(device));
     LS_CHECK(this, hipDeviceSynchronize());
     int cnt[4] = {0, 0, 0, 0};
@@ -673,15 +675,12 @@ linon_status Context::surrogate_collect_stats() {
         float ms[5];
         for (int k = 0; k < 5; ++k) (void)hipEventElapsedTime(&ms[k], e[k], e[k + 1]);
         F2.approach_ms = ms[0];
-        F2.mlp_ms = ms[1];  // coarse flow map + MLP + verdicts
         F2.fallback_ms = ms[2];
         F2.overlay_ms = ms[3];
         F2.rejudge_ms = ms[4];
-        F2.frame_ms = ms[0] + ms[1] + ms[2] + ms[3];
+        F2.frame_ms = ms[0] + ms[1] + ms[2] + ms[3];  // ms[1]: coarse flow map + MLP + verdicts
         (void)hipEventElapsedTime(&F2.coarse_ms, e[1], e[6]);
-        float mlp = 0.0f;
-        (void)hipEventElapsedTime(&mlp, e[6], e[2]);
-        F2.mlp_ms = mlp;
+        (void)hipEventElapsedTime(&F2.mlp_ms, e[6], e[2]);
     }
     if (cnt[3] > 0) {
         std::vector<float> rj(cnt[3]);
@@ -693,7 +692,6 @@ linon_status Context::surrogate_collect_stats() {
         F2.rejudge_frac_within_tol =
             (float)(std::upper_bound(rj.begin(), rj.end(), S.cfg.tolerance) - rj.begin()) / (float)rj.size();
     }
-    S.stats_stale = false;
     return LINON_OK;
 }
 

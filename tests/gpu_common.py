@@ -181,3 +181,38 @@ def entry_states(n, seed, radius=2.0, inset=0.01):
     q /= np.linalg.norm(q, axis=1, keepdims=True)
     q[(p * q).sum(1) > 0] *= -1.0
     return np.concatenate([p, q], 1).astype(np.float32)
+
+
+class HipGraph:
+    """A HIP graph captured on a new stream through the HIP runtime the library runs on (no
+    torch: a second runtime in the process would see no device once the library holds it).
+    `frames` are callables that enqueue work on the stream handle they are given."""
+
+    def __init__(self, frames):
+        import ctypes as C
+
+        self.hip = hip = C.CDLL("libamdhip64.so.7")
+        self.stream, self.graph, self.exe = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        self._ok(hip.hipStreamCreate(C.byref(self.stream)), "hipStreamCreate")
+        self._ok(hip.hipStreamBeginCapture(self.stream, 1), "hipStreamBeginCapture")  # thread-local mode
+        try:
+            for frame in frames:
+                frame(self.stream.value)
+        finally:
+            self._ok(hip.hipStreamEndCapture(self.stream, C.byref(self.graph)), "hipStreamEndCapture")
+        self._ok(hip.hipGraphInstantiateWithFlags(C.byref(self.exe), self.graph, C.c_ulonglong(0)),
+                 "hipGraphInstantiateWithFlags")
+
+    @staticmethod
+    def _ok(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed: {rc}")
+
+    def replay(self):
+        self._ok(self.hip.hipGraphLaunch(self.exe, self.stream), "hipGraphLaunch")
+        self._ok(self.hip.hipStreamSynchronize(self.stream), "hipStreamSynchronize")
+
+    def close(self):
+        self.hip.hipGraphExecDestroy(self.exe)
+        self.hip.hipGraphDestroy(self.graph)
+        self.hip.hipStreamDestroy(self.stream)

@@ -8,10 +8,10 @@ import numpy as np
 import pytest
 
 import linon_amd as L
-from linon_amd import configs
+from linon_amd import configs, scenes
 from oracle import oracle as O
 
-from gpu_common import EXIT_RTOL, PROBE_VARIANTS, compare, entry_states, rel_err, run_oracles
+from gpu_common import EXIT_RTOL, PROBE_VARIANTS, HipGraph, compare, entry_states, rel_err, run_oracles
 
 pytestmark = pytest.mark.gpu
 
@@ -298,3 +298,56 @@ def test_multifidelity_surrogate_accepts_rays_within_tolerance(trained_coarse):
     assert (rel[acc] <= cfg.tolerance).mean() >= 0.99, np.quantile(rel[acc], [0.5, 0.99, 1.0])
     assert st["rejudged"] > 0 and st["rejudge_err_q99"] <= cfg.tolerance
     assert (rel[~acc] <= 1e-4).all()
+
+
+def test_plain_and_surrogate_frames_in_one_graph(trained_coarse):
+    """A caller captures a plain frame of one view (mesh, overlay) and the surrogate + fallback
+    frame of another view into one HIP graph on its own stream.  With eager frames of both views
+    between the replays, every replay leaves both views' outputs as their eager frames left
+    them, bit for bit, and the surrogate's counts as the eager frame's (they stay on the
+    device, so the replay rewrites them)."""
+    sview, cfg, _ = trained_coarse
+    cam = L.Camera.look_at((0.2, 0.3, 3.0), (0, 0, 0), fov_deg=60.0)
+    sview.update_camera(cam)
+    sview.update_settings(L.Settings.make(overlay_mode=1))
+    sview.set_integrator(L.Integrator.make(h_initial=cfg.h, t_max=1e30, adaptive=False, max_steps=cfg.max_steps,
+                                           mode=L.MODE_ESCAPE, escape_radius=cfg.radius, samples_per_ray=0,
+                                           sample_steps=0, need_mapping=True, clip_exit=True))
+    sview.set_field("swirl")
+    sview.render_surrogate()
+    want_s = sview.read_mapping()
+    want_acc = sview.read_surrogate_accepted()
+    want_st = sview.surrogate_stats()
+    assert want_st["accepted"] > 0
+    v, f = scenes.cornell_box_centered()
+    with L.MainView(64, 48) as pv:
+        pv.update_model(v, f)
+        pv.update_camera(L.Camera.look_at((0.0, 0.0, 3.0), (0, 0, 0), fov_deg=60.0))
+        pv.update_settings(L.Settings.make(overlay_mode=1))
+        pv.set_integrator(L.Integrator.make())
+        pv.set_field("lens")
+        pv.render()
+        want_p = pv.read_frame()
+        print("[case] plain + surrogate frames in one graph", flush=True)
+        g = HipGraph([lambda st: pv.render(stream=st), lambda st: sview.render_surrogate(stream=st)])
+        try:
+            for rep in range(3):
+                pv.set_field("swirl")            # eager work on both views between the replays
+                pv.render()
+                sview.render()                   # the plain RK4 frame: other exits where accepted
+                assert not np.array_equal(sview.read_mapping().view(np.uint32), want_s.view(np.uint32))
+                pv.set_field("lens")
+                g.replay()
+                got = pv.read_frame()
+                assert np.array_equal(got.mapping.view(np.uint32), want_p.mapping.view(np.uint32)), rep
+                assert np.array_equal(got.final, want_p.final), rep
+                assert np.array_equal(got.hit_faces, want_p.hit_faces), rep
+                assert np.array_equal(sview.read_mapping().view(np.uint32), want_s.view(np.uint32)), rep
+                assert np.array_equal(sview.read_surrogate_accepted(), want_acc), rep
+                st = sview.surrogate_stats()
+                for k in ("entered", "accepted", "conf_fallback", "lyapunov_fallback"):
+                    assert st[k] == want_st[k], (rep, k)
+                # the re-judged sample is drawn afresh per frame (the capture is a later frame)
+                assert st["rejudged"] > 0 and st["rejudge_err_q99"] <= cfg.tolerance, st
+        finally:
+            g.close()
