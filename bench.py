@@ -501,6 +501,8 @@ def bench_batch(verts, faces, replays: int) -> dict:
     rays = sum(v.width * v.height for v in views)
     # the same views one after another through the plain kernel (escape mode, same steps)
     plain_ms = 0.0
+    per_field = {}
+    names = {v: k for k, v in L.FIELDS.items()}
     ig = configs.batch_integrator(max_steps=8000)
     for bv in views:
         view.resize_texture(bv.width, bv.height)
@@ -511,7 +513,15 @@ def bench_batch(verts, faces, replays: int) -> dict:
         view.set_field(bv.field)
         view.reset_timings()
         view.render()
-        plain_ms += float(view.timings()[0][0])
+        ms1 = float(view.timings()[0][0])
+        plain_ms += ms1
+        pf = per_field.setdefault(names[bv.field], {"views": 0, "rays": 0, "rk4_steps": 0, "plain_sequential_ms": 0.0})
+        pf["views"] += 1
+        pf["rays"] += bv.width * bv.height
+        pf["rk4_steps"] += view.counters()["rk4_steps"]
+        pf["plain_sequential_ms"] += ms1
+    for pf in per_field.values():
+        pf["plain_gsteps_per_s"] = pf["rk4_steps"] / (pf["plain_sequential_ms"] * 1e-3) / 1e9
     view.close()
     return {"workload": "65 views (seed 0): resolutions {256^2..2048^2}, eye radius U(2.5,4), fields swirl/lens/"
                         "hot-air/warp, h in {0.01,0.005,0.0025} + one Enhance view at h/4; escape mode, max 8000",
@@ -519,6 +529,7 @@ def bench_batch(verts, faces, replays: int) -> dict:
             "wall_ms": wall * 1e3, "gsteps_per_s": c["rk4_steps"] / (dev_ms * 1e-3) / 1e9,
             "mrays_per_s": rays / (dev_ms * 1e-3) / 1e6, "plain_kernel_sequential_ms": plain_ms,
             "speedup_vs_plain_sequential": plain_ms / dev_ms if dev_ms > 0 else None,
+            "per_field": per_field,
             "tuning": tuning.as_dict(), "autotune_full_size_ms": tuned_ms, "autotune_seconds": tune_s,
             "accuracy": accuracy}
 

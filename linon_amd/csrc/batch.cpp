@@ -108,14 +108,27 @@ linon_status Context::batch_prepare(BatchState& B, const linon_view* views, int 
     return LINON_OK;
 }
 
-// Queue resets (fill kernels, so the sequence can be captured) and one persistent launch per
-// field group, all on stream s.
+// Queue resets (fill kernels, so the sequence can be captured) on stream s, then one persistent
+// launch per field group: the first on s, the others on side streams forked from s and joined
+// back into it, so the groups run concurrently and one group's tail (its last, longest rays)
+// overlaps the next group's work instead of idling the device.  The groups share nothing but
+// the counters (atomics) and their own queues.
 linon_status Context::batch_launch(BatchState& B, const linon_batch_tuning& T, hipStream_t s) {
     const int cus = std::max(1, num_cus);
     const int wpb = std::max(1, std::min(4, T.waves_per_block));
     const unsigned blocks = (unsigned)std::max(1, cus * std::max(1, T.waves_per_cu) / wpb);
     LB_CHECK(this, launch_clear_u64(B.d_queues, (int)B.groups.size(), s));
     LB_CHECK(this, launch_clear_u64(d_counters, LINON_NUM_COUNTERS, s));
+    const int nside = (int)std::min<size_t>(3, B.groups.size() > 0 ? B.groups.size() - 1 : 0);
+    for (int k = 0; k < nside; ++k) {
+        if (!batch_side[k]) LB_CHECK(this, hipStreamCreateWithFlags(&batch_side[k], hipStreamNonBlocking));
+        if (!batch_join[k]) LB_CHECK(this, hipEventCreateWithFlags(&batch_join[k], hipEventDisableTiming));
+    }
+    if (nside > 0) {
+        if (!batch_fork) LB_CHECK(this, hipEventCreateWithFlags(&batch_fork, hipEventDisableTiming));
+        LB_CHECK(this, hipEventRecord(batch_fork, s));
+        for (int k = 0; k < nside; ++k) LB_CHECK(this, hipStreamWaitEvent(batch_side[k], batch_fork, 0));
+    }
     SceneParams S;
     fill_scene(S);
     for (size_t gi = 0; gi < B.groups.size(); ++gi) {
@@ -131,8 +144,14 @@ linon_status Context::batch_launch(BatchState& B, const linon_batch_tuning& T, h
         P.refill_min = std::max(1, std::min(64, T.refill_min));
         P.step = B.step;
         P.scene = S;
-        LB_CHECK(this, wgsl_math_for(g.field) ? linon_wgslmath::launch_batch(g.field, P, blocks, 64u * (unsigned)wpb, s)
-                                               : launch_batch(g.field, P, blocks, 64u * (unsigned)wpb, s));
+        // groups 1, 2, 3, 4, ... go to side streams 0, 1, 2, 0, ... (the first stays on s)
+        hipStream_t gs = gi == 0 ? s : batch_side[(gi - 1) % (size_t)nside];
+        LB_CHECK(this, wgsl_math_for(g.field) ? linon_wgslmath::launch_batch(g.field, P, blocks, 64u * (unsigned)wpb, gs)
+                                               : launch_batch(g.field, P, blocks, 64u * (unsigned)wpb, gs));
+    }
+    for (int k = 0; k < nside; ++k) {
+        LB_CHECK(this, hipEventRecord(batch_join[k], batch_side[k]));
+        LB_CHECK(this, hipStreamWaitEvent(s, batch_join[k], 0));
     }
     return LINON_OK;
 }

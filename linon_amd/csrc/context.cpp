@@ -409,6 +409,11 @@ void linon_destroy(linon_context* ctx) {
     if (c->d_outline) (void)hipFree(c->d_outline);
     for (auto& e : c->ring)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->batch_join)
+        if (e) (void)hipEventDestroy(e);
+    if (c->batch_fork) (void)hipEventDestroy(c->batch_fork);
+    for (auto& s : c->batch_side)
+        if (s) (void)hipStreamDestroy(s);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

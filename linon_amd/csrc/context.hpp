@@ -92,6 +92,11 @@ struct Surrogate {
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // side streams and events that let a batch's field groups run concurrently (created on
+    // first use; the fork and joins are captured into the batch's graph as edges)
+    hipStream_t batch_side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t batch_fork = nullptr;
+    hipEvent_t batch_join[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     bool timed = false;
     // per-render event triples (start, after ray cast, after overlay), a ring of slots

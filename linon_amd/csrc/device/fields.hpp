@@ -90,7 +90,7 @@ struct FieldSwirl {  // rotateY(v0, 0.5 * length(p))
 struct FieldLens {  // v - 0.3 * p / max(|p|^3, 1e-3)
     __device__ __forceinline__ vec3 operator()(vec3, vec3 p, vec3, vec3 v, float) const {
         float r = length(p);
-        return v - div3_rn_bounded(0.3f * p, max(r * r * r, 0.001f));  // 0.3 p / max(r^3, 0.001)
+        return v - div3_rn_lens(0.3f * p, max(r * r * r, 0.001f));  // 0.3 p / max(r^3, 0.001)
     }
 };
 struct FieldHotAir {  // refraction(T(p_prev), T(p), v, normalize(p - c)), T = 20 + 80 sigmoid(8 (0.5 - |p - c|))

@@ -95,6 +95,8 @@ __device__ __forceinline__ float sqrt_rn(float x) {
 #ifdef LINON_FAST_MATH
     return __builtin_amdgcn_sqrtf(x);
 #else
+    // branch-free: a branch around the rare rescaling would end the basic block, and the step
+    // loop then loses the scheduling that hides the root's latency (about 1.5x slower)
 #pragma clang fp contract(off)
     const bool tiny = x < 0x1p-96f;
     const float xs = tiny ? x * 0x1p32f : x;
@@ -132,6 +134,46 @@ __device__ __forceinline__ vec3 div3_rn_bounded(vec3 v, float s) {
     if (s <= 0x1p60f && lo >= 0x1p-60f) {
         const float y = recip_rn(s);
         return {quot_rn(v.x, s, y), quot_rn(v.y, s, y), quot_rn(v.z, s, y)};
+    }
+    return {v.x / s, v.y / s, v.z / s};
+#endif
+}
+// Markstein's correction written with the residual's sign flipped, r' = RN(s q0 - a),
+// q = RN(q0 - r' y): for a nonzero a, r' is -r exactly and q the same rounding as
+// RN(q0 + r y) (quot_rn); for a = +-0 it keeps the zero's sign (q0 = +-0, r' = +0, q = +-0),
+// which the unflipped form loses for -0.
+__device__ __forceinline__ float quot_rn_signed_zero(float a, float b, float y) {
+#pragma clang fp contract(off)
+    const float q0 = a * y;
+    const float rn = __builtin_fmaf(b, q0, -a);
+    return __builtin_fmaf(-rn, y, q0);
+}
+// The lens field's 0.3 p / s with s = max(|p|^3, 0.001) >= 0.001 (or NaN), where every |0.3 p_i|
+// is at most 2^61 while s <= 2^60.  Its captured rays sit at p = 0 or shrink towards it through
+// ever smaller coordinates, which div3_rn_bounded sends to the IEEE division.  Here a component
+// may be zero (signed-zero form above) or as small as 2^-102 while s <= 2^22.  The quotient
+// a / s is then at least 2^-125, a normal number, so q0 = RN(a y) is within an ulp of it as in
+// the normal range; a and s q0 are multiples of 2^(e_a - 23) and 2^(e_a - 47) >= 2^-149 (the
+// exponents of s and q0 add up to at least e_a - 1), so the exact residual a - s q0, which the
+// normal-range argument shows to need at most 24 significant bits, is also a multiple of 2^-149
+// and the FMA forms it exactly even where it is subnormal; the corrected quotient is then the
+// correctly rounded one, as in the normal range.  For s in (2^22, 2^60] the bound stays 2^-60
+// (div3_rn_bounded's); anything else takes the ordinary division.  The test orders magnitudes
+// by their bit patterns doubled minus 2, which sends both zeros to the top.  Same result as
+// v / s for every finite p.
+__device__ __forceinline__ vec3 div3_rn_lens(vec3 v, float s) {
+#ifdef LINON_FAST_MATH
+    return v / s;
+#else
+    const unsigned ux = (__float_as_uint(v.x) << 1) - 2u;
+    const unsigned uy = (__float_as_uint(v.y) << 1) - 2u;
+    const unsigned uz = (__float_as_uint(v.z) << 1) - 2u;
+    const unsigned m = min(min(ux, uy), uz);
+    const unsigned lo102 = (0x0c800000u << 1) - 2u;  // 2^-102
+    const unsigned lo60 = (0x21800000u << 1) - 2u;   // 2^-60
+    if ((s <= 0x1p22f && m >= lo102) || (s <= 0x1p60f && m >= lo60)) {
+        const float y = recip_rn(s);
+        return {quot_rn_signed_zero(v.x, s, y), quot_rn_signed_zero(v.y, s, y), quot_rn_signed_zero(v.z, s, y)};
     }
     return {v.x / s, v.y / s, v.z / s};
 #endif

@@ -158,6 +158,35 @@ __global__ void __launch_bounds__(256) div_selftest_mantissa_kernel(int ea, floa
     if (bad) atomicAdd(mism, (unsigned long long)bad);
 }
 
+// The bounded divisions (div3_rn_bounded: normalize; div3_rn_lens: the lens field) with every
+// numerator mantissa at exponent ea (ea = -127: the subnormals), negated, and beside a zero of
+// each sign and 0.5, against one denominator s, component-wise against IEEE division; the
+// numerators stay below 2^61 as their callers guarantee
+template <int Which>
+__device__ __forceinline__ lw::vec3 bounded_div(lw::vec3 v, float s) {
+    return Which == 0 ? lw::div3_rn_bounded(v, s) : lw::div3_rn_lens(v, s);
+}
+template <int Which>
+__device__ __forceinline__ unsigned bounded_mismatches(float a, float s) {
+    unsigned bad = 0;
+    const lw::vec3 q1 = bounded_div<Which>(lw::vec3f(a, -a, 0.0f), s);
+    bad += fbits(q1.x) != fbits(a / s);
+    bad += fbits(q1.y) != fbits(-a / s);
+    bad += fbits(q1.z) != fbits(0.0f / s);
+    const lw::vec3 q2 = bounded_div<Which>(lw::vec3f(-0.0f, 0.5f, a), s);
+    bad += fbits(q2.x) != fbits(-0.0f / s);
+    bad += fbits(q2.y) != fbits(0.5f / s);
+    bad += fbits(q2.z) != fbits(a / s);
+    return bad;
+}
+__global__ void __launch_bounds__(256) div_selftest_bounded_kernel(int ea, float s, unsigned long long* mism) {
+    const unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (1u << 23)) return;
+    const float a = __uint_as_float(((unsigned)(ea + 127) << 23) | m);
+    const unsigned bad = bounded_mismatches<0>(a, s) + bounded_mismatches<1>(a, s);
+    if (bad) atomicAdd(mism, (unsigned long long)bad);
+}
+
 // random operands (SplitMix64 stream): sign, exponent in [-70, 70] (so the out-of-range and
 // zero paths are taken too), mantissa uniform
 __device__ __forceinline__ unsigned long long smix(unsigned long long z) {
@@ -331,6 +360,19 @@ hipError_t run_division_selftest(unsigned long long random_pairs, unsigned long 
             if (e != hipSuccess) break;
             hipLaunchKernelGGL(div_selftest_mantissa_kernel, dim3(mblocks), dim3(256), 0, nullptr, ea, b, d + 1);
             n += 3ull << 23;
+        }
+    // the bounded division (normalize, the lens field) for zero, tiny and subnormal numerators:
+    // every mantissa at exponents from the subnormals up through its fast range's lower end,
+    // against denominators across [2^-61, 2^60], the lens field's floor 0.001 and beyond
+    const float bdenoms[] = {0.001f, 0.3f, 1.0f, 1.99999988f, 7.77f, 1000.0f, 0x1p-61f, 0x1.8p-61f, 0x1p-40f,
+                             0x1p60f, 0x1.fffffep59f, 0x1p61f, 0x1p-62f, 3.0e-20f, 0x1p22f, 0x1.fffffep21f,
+                             0x1.000002p22f, 0x1.8p21f, 0x1.fffffep-1f};
+    for (float b : bdenoms)
+        for (int ea : {-127, -126, -125, -110, -104, -103, -102, -101, -100, -90, -80, -70, -62, -61, -60, -59, -30,
+                       0}) {
+            if (e != hipSuccess) break;
+            hipLaunchKernelGGL(div_selftest_bounded_kernel, dim3(mblocks), dim3(256), 0, nullptr, ea, b, d + 1);
+            n += 12ull << 23;
         }
     for (unsigned hi = 0; hi < 256 && e == hipSuccess; ++hi) {  // sigmoid_floor at every float
         hipLaunchKernelGGL(div_selftest_sigmoid_kernel, dim3((1u << 24) / 256), dim3(256), 0, nullptr, hi, d + 1);

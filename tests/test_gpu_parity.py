@@ -349,7 +349,10 @@ def test_shared_denominator_division_is_ieee():
     (zeros, out-of-range and signed operands included), sigmoid_floor at every float and
     refraction_in_range on 2^28 random temperature pairs and directions.  On the same draws, the
     first RK4 stage of hot air and the four mirages, which skips the index ratio n(T)/n(T) = 1,
-    must equal the field evaluated at p_prev = p."""
+    must equal the field evaluated at p_prev = p.  The bounded divisions of normalize and of
+    the lens field (whose relaxed range admits zeros and numerators down to 2^-102) are also
+    checked at every numerator mantissa of 18 exponents from the subnormals up to 2^0, beside
+    signed zeros, against 19 denominators in and out of their ranges."""
     print("[case] division self-test", flush=True)
     res = L.selftest_division(random_pairs=1 << 28)
     print(res, flush=True)
