@@ -588,7 +588,8 @@ def surrogate_leg(field: str, steps: int, wl=configs.OVERLAY_1080P, train: bool 
            "rejudged": int(st["rejudged"]), "rejudge_err_q50": st["rejudge_err_q50"],
            "rejudge_err_q99": st["rejudge_err_q99"], "rejudge_err_max": st["rejudge_err_max"],
            "phases_ms": {k: st[k] for k in ("approach_ms", "coarse_ms", "mlp_ms", "fallback_ms", "overlay_ms")},
-           # algorithmic MLP FLOP per ray: 2 x (198*256 + 3*256*256 + 256*7) = 498,176
+           # the fused MLP kernel's rate (mlp_ms times that kernel alone, with HIP events on the
+           # frame's stream); algorithmic FLOP per ray: 2 x (198*256 + 3*256*256 + 256*7) = 498,176
            "mlp_tflops": (int(st["entered"]) * 498176 / (st["mlp_ms"] * 1e-3) / 1e12
                           if st["mlp_ms"] > 0 else None)}
     log(f"surrogate {field} {wl.width}x{wl.height}: {res['speedup']}x, accepted "

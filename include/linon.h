@@ -176,6 +176,8 @@ typedef struct linon_surrogate_stats {
     float holdout_err_q50, holdout_err_q99, holdout_frac_within_tol;
     uint64_t frame_rays, entered, accepted, conf_fallback, lyapunov_fallback, rejudged;
     float rejudge_err_q50, rejudge_err_q99, rejudge_err_max, rejudge_frac_within_tol;
+    /* frame phases: approach RK4; the fused MLP kernel alone (mlp_ms); RK4 fallback; overlay and
+     * Lyapunov gate; re-judging (not in frame_ms) */
     float frame_ms, approach_ms, mlp_ms, fallback_ms, overlay_ms, rejudge_ms;
     float coarse_ms;  /* the coarse flow map of the entered rays (frame_ms includes it) */
 } linon_surrogate_stats;

@@ -83,7 +83,7 @@ struct Surrogate {
     unsigned char* f_mask = nullptr;  // per texel: the frame kept the surrogate's exit
     linon_surrogate_stats train_stats{};
     linon_surrogate_stats frame_stats{};
-    hipEvent_t ev[7] = {};      // frame phase boundaries
+    hipEvent_t ev[8] = {};      // frame phase boundaries
     bool frame_timed = false;   // the last frame run from the host recorded them (it was not captured)
     bool has_frame = false;     // a frame has been rendered or captured
     void release();

@@ -541,7 +541,7 @@ linon_status Context::surrogate_frame(hipStream_t s) {
     }
     if (!S.ev[0])
         for (auto& x : S.ev) LS_CHECK(this, hipEventCreate(&x));
-    hipEvent_t* e = S.ev;  // e[6]: after the coarse flow map, inside the MLP phase
+    hipEvent_t* e = S.ev;  // e[6], e[7]: around the MLP kernel, inside the MLP phase
     auto mark = [&](int k) { return capturing ? hipSuccess : hipEventRecord(e[k], s); };
     // device counts: [0] entered, [1] confidence fallback, [2] Lyapunov fallback, [3] re-judged
     int* const n_in = S.f_count;
@@ -598,6 +598,7 @@ linon_status Context::surrogate_frame(hipStream_t s) {
     M.b_out = S.b[4];
     M.out = S.f_out;
     LS_CHECK(this, sg_mlp_infer(M, s));
+    LS_CHECK(this, mark(7));
     LS_CHECK(this, sg_accept(S.f_out, coarse ? S.f_base : nullptr, S.f_texel, npx, n_in, S.scale_p,
                              S.cfg.conf_threshold, d_mapping, S.f_pred, S.f_list, n_fb, S.f_accepted, s));
     LS_CHECK(this, mark(2));
@@ -680,7 +681,7 @@ linon_status Context::surrogate_collect_stats() {
         F2.rejudge_ms = ms[4];
         F2.frame_ms = ms[0] + ms[1] + ms[2] + ms[3];  // ms[1]: coarse flow map + MLP + verdicts
         (void)hipEventElapsedTime(&F2.coarse_ms, e[1], e[6]);
-        (void)hipEventElapsedTime(&F2.mlp_ms, e[6], e[2]);
+        (void)hipEventElapsedTime(&F2.mlp_ms, e[6], e[7]);  // the fused MLP kernel alone
     }
     if (cnt[3] > 0) {
         std::vector<float> rj(cnt[3]);
