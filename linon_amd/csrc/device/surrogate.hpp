@@ -119,6 +119,10 @@ struct FlowmapParams {
     const int* n_dev;       // optional: the count on the device (a frame's compacted lists)
     float* exits;           // [n or max index][6] exit point and direction, at the state's slot
     int* steps;             // optional, RK4 iterations per state slot (added to what is there)
+    // optional (a frame's fallback): the RK4 iterations the ray already ran before its entry
+    // state, iters_at[texel_of[slot]], which count toward max_steps as in one plain RK4 run
+    const int* iters_at;
+    const int* texel_of;
 };
 
 // RK4 from an entry state to the sphere exit (the surrogate's training labels, and the
@@ -131,6 +135,8 @@ __device__ void flowmap_body(const FlowmapParams& P) {
     const float* s = P.states + 6 * (size_t)slot;
     lk::RayState r;
     lk::ray_begin(r, vec3f(s[0], s[1], s[2]), vec3f(s[3], s[4], s[5]), P.step);
+    const int before = P.iters_at ? P.iters_at[P.texel_of[slot]] : 0;
+    r.iters = before;
     SceneParams none;
     none.accel = LINON_ACCEL_BRUTE_K;
     none.num_faces = 0;
@@ -141,7 +147,7 @@ __device__ void flowmap_body(const FlowmapParams& P) {
     float* o = P.exits + 6 * (size_t)slot;
     o[0] = r.p.x; o[1] = r.p.y; o[2] = r.p.z;
     o[3] = r.d.x; o[4] = r.d.y; o[5] = r.d.z;
-    if (P.steps) P.steps[slot] += r.iters;
+    if (P.steps) P.steps[slot] += r.iters - before;
 }
 
 }  // namespace ls

@@ -611,6 +611,9 @@ linon_status Context::surrogate_frame(hipStream_t s) {
     F.index = S.f_list;
     F.n = npx;
     F.n_dev = n_fb;
+    // the approach's iterations count toward max_steps, as in the plain frame's one RK4 run
+    F.iters_at = d_steps;
+    F.texel_of = S.f_texel;
     LS_CHECK(this, flowmap(F));
     LS_CHECK(this, sg_apply_rk4(S.f_list, npx, n_fb, S.f_exits, S.f_texel, S.f_pred, d_mapping, nullptr, 1, s));
     LS_CHECK(this, mark(3));

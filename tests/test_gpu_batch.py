@@ -218,3 +218,32 @@ def test_batch_views_against_the_oracle():
         assert (got[i].steps == ref["steps"]).mean() >= 0.99
         fields.add(bv.field)
     assert len(fields) == 4
+
+
+def test_the_65_view_benchmark_batch():
+    """The benchmark's own batch (64 views drawn from seed 0 plus the Enhance view: resolutions
+    up to 2048^2, the four fields, h in {0.01, 0.005, 0.0025} and h/4; escape mode, at most 8000
+    RK4 steps; the 20,516-face scene) in one graph replay.  Its ray count is every view's
+    pixels, and a sample of its views -- view 0 and its Enhance copy, the first view of each
+    field and the largest view -- equals the plain kernel's frames bit for bit."""
+    wl = configs.OVERLAY_1080P
+    v, f = wl.mesh()
+    views = configs.batch_views(num_views=64, seed=0)
+    names = {val: k for k, val in L.FIELDS.items()}
+    pick = {0, len(views) - 1, max(range(len(views)), key=lambda i: views[i].width * views[i].height)}
+    for name in ("swirl", "lens", "hot_air", "warp"):
+        pick.add(next(i for i, bv in enumerate(views) if names[bv.field] == name))
+    with L.MainView(64, 64) as view:
+        view.update_model(v, f)
+        view.set_integrator(configs.batch_integrator(max_steps=8000))
+        print(f"[case] 65-view benchmark batch, views {sorted(pick)} checked", flush=True)
+        view.set_batch(views)
+        view.render_batch()
+        c = view.counters()
+        assert c["rays"] == sum(bv.width * bv.height for bv in views)
+        got = {i: view.read_batch_view(i) for i in sorted(pick)}
+        for i in sorted(pick):
+            ref = _plain(view, views[i], max_steps=8000)
+            assert np.array_equal(got[i].hit_faces, ref.hit_faces), i
+            assert np.array_equal(got[i].mapping.view(np.uint32), ref.mapping.view(np.uint32)), i
+            assert np.array_equal(got[i].steps, ref.steps), i

@@ -351,3 +351,43 @@ def test_plain_and_surrogate_frames_in_one_graph(trained_coarse):
                 assert st["rejudged"] > 0 and st["rejudge_err_q99"] <= cfg.tolerance, st
         finally:
             g.close()
+
+
+@pytest.mark.parametrize("field", ["swirl", "lens", "hot_air", "warp"])
+def test_saved_surrogates_at_1080p(field):
+    """The saved surrogate weights the bench loads, on the 1080p workload's view without the mesh:
+    swirl, warp and hot air accept most of the rays that enter the sphere (lens's captured rays
+    leave the network unconfident, and it may accept none), >= 99% of the accepted exits are
+    within the 1e-3 tolerance of the plain RK4 escape-mode frame, the re-judged sample's q99 is
+    within it, and every texel that did not take the surrogate's exit is the plain frame's to
+    1e-4 (the surrogate's RK4 legs round as the plain kernel does)."""
+    wl = configs.OVERLAY_1080P
+    path = configs.saved_surrogate(field, wl)
+    assert path is not None, f"no saved weights for {field}"
+    with L.MainView(wl.width, wl.height) as view:
+        view.update_model(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
+        view.update_camera(wl.camera())
+        view.update_settings(wl.settings())
+        view.load_surrogate(path)
+        view.set_integrator(L.Integrator.make(h_initial=wl.h, t_max=1e30, adaptive=False, max_steps=wl.max_steps,
+                                              mode=L.MODE_ESCAPE, escape_radius=2.0, samples_per_ray=0,
+                                              sample_steps=0, need_mapping=True, clip_exit=True))
+        view.set_field(field)
+        print(f"[case] saved surrogate {field} at 1080p", flush=True)
+        view.render()
+        plain = view.read_mapping()
+        view.render_surrogate()
+        st = view.surrogate_stats()
+        sur = view.read_mapping()
+        acc = view.read_surrogate_accepted()
+    print({k: st[k] for k in ("entered", "accepted", "conf_fallback", "lyapunov_fallback", "rejudged",
+                              "rejudge_err_q99")}, flush=True)
+    assert st["accepted"] == int(acc.sum())
+    rel = rel_err(sur, plain)
+    assert (rel[~acc] <= 1e-4).mean() >= 0.999
+    if field == "lens":
+        assert st["accepted"] == 0 or (rel[acc] <= 1e-3).mean() >= 0.99
+        return
+    assert st["accepted"] > 0.6 * st["entered"] > 0
+    assert (rel[acc] <= 1e-3).mean() >= 0.99, np.quantile(rel[acc], [0.5, 0.99, 1.0])
+    assert st["rejudged"] > 0 and st["rejudge_err_q99"] <= 1e-3
