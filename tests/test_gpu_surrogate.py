@@ -391,3 +391,24 @@ def test_saved_surrogates_at_1080p(field):
     assert st["accepted"] > 0.6 * st["entered"] > 0
     assert (rel[acc] <= 1e-3).mean() >= 0.99, np.quantile(rel[acc], [0.5, 0.99, 1.0])
     assert st["rejudged"] > 0 and st["rejudge_err_q99"] <= 1e-3
+
+
+def test_surrogate_frame_with_no_entered_rays(trained_coarse):
+    """A camera that looks away from the bounding sphere: no ray enters, every launch of the
+    frame reads a zero count on the device and does nothing, and the mapping is the plain
+    escape-mode frame's."""
+    view, cfg, _ = trained_coarse
+    view.update_camera(L.Camera.look_at((0.0, 0.0, 3.0), (0.0, 0.0, 6.0), fov_deg=30.0))
+    view.update_settings(L.Settings.make(overlay_mode=1))
+    view.set_integrator(L.Integrator.make(h_initial=cfg.h, t_max=1e30, adaptive=False, max_steps=cfg.max_steps,
+                                          mode=L.MODE_ESCAPE, escape_radius=cfg.radius, samples_per_ray=0,
+                                          sample_steps=0, need_mapping=True, clip_exit=True))
+    view.set_field("swirl")
+    print("[case] surrogate frame, no ray enters", flush=True)
+    view.render()
+    plain = view.read_mapping()
+    view.render_surrogate()
+    st = view.surrogate_stats()
+    assert st["entered"] == 0 and st["accepted"] == 0 and st["rejudged"] == 0, st
+    assert not view.read_surrogate_accepted().any()
+    assert np.array_equal(view.read_mapping().view(np.uint32), plain.view(np.uint32))
