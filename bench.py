@@ -153,6 +153,32 @@ def bench_plain_256(reps: int = 5) -> dict:
             "gsteps_per_s": c["rk4_steps"] / (r_ms * 1e-3) / 1e9, "accuracy": acc}
 
 
+def bench_adaptive(view, wl, verts, faces) -> dict:
+    """The headline frames once more with the reference's adaptive step rule on (a redo at
+    h = 0.001 when |V - last V| jumps tenfold, back to h after it calms; the shader's default,
+    which the configurations' fixed step turns off): device time, RK4 steps (redos included)
+    and the error of 256 random pixels against the FP32 and FP64 oracles, per field."""
+    import dataclasses
+
+    wla = dataclasses.replace(wl, adaptive=True)
+    view.set_integrator(wla.integrator())
+    out = {"workload": wl.name + " with adaptive steps", "per_field": {}}
+    for name in wl.fields:
+        view.set_field(name)
+        view.render()
+        c = view.counters()
+        view.reset_timings()
+        view.render()
+        r_ms = float(view.timings()[0][0])
+        fr = view.read_frame()
+        out["per_field"][name] = {
+            "raycast_ms": r_ms, "rk4_steps": c["rk4_steps"], "gsteps_per_s": c["rk4_steps"] / (r_ms * 1e-3) / 1e9,
+            "accuracy": sample_accuracy(wla.camera(), wla.settings(), wla.integrator(), name, verts, faces,
+                                        wla.width, wla.height, fr.mapping, fr.hit_faces, 256, seed=4)}
+    view.set_integrator(wl.integrator())
+    return out
+
+
 def cpu_baseline(wl, verts, faces, budget_s: float, frames: dict | None = None, accuracy_pixels: int = 200_000) -> dict:
     """FP32 CPU restatement on a random sample of each field's pixels, timed with the reference's
     own scene loop: every face tested at every pre-hit step, in index order (the oracle's exact
@@ -338,8 +364,10 @@ def main() -> int:
         "per_field": per_field,
         "roofline": roofline,
     }
-    view.close()
     extra = {}
+    log("adaptive 1080p ...")
+    extra["adaptive_1080p"] = bench_adaptive(view, wl, verts, faces)
+    view.close()
     log("plain 256x256 ...")
     extra["plain_256_swirl"] = bench_plain_256()
     if not a.skip_batch:
