@@ -473,9 +473,12 @@ __device__ __forceinline__ bool ray_finished(const RayState& r, const StepParams
 // field_weight, the segment query until the first hit, and the escape tests of the escape
 // mode.  Returns true once the ray is finished.  `lds` holds the faces when the brute-force
 // path staged them, else null; the scene queries are counted into `st`.
-template <class F>
-__device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, const SceneParams& S,
-                                            const float4* lds, int budget, QueryStats& st) {
+// WOne: the field weight is 1 (every benchmark configuration).  The blend (1 - w) d + w V then
+// is RN(RN(0 d) + V), which one FMA fma(0, d, V) rounds identically (0 d is exact), so the
+// step loop saves two of its three operations per component.
+template <class F, bool WOne>
+__device__ __forceinline__ bool ray_advance_impl(RayState& r, const StepParams& I, const SceneParams& S,
+                                                 const float4* lds, int budget, QueryStats& st) {
     const F field{};
     const bool adaptive = I.adaptive != 0;
     const bool escape = (I.mode == LINON_MODE_ESCAPE);
@@ -506,7 +509,11 @@ __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, co
                 continue;
             }
         }
-        r.d = rn::add(rn::mul(rn::sub(1.0f, w), r.d), rn::mul(w, V));
+        if (WOne)
+            r.d = vec3f(__builtin_fmaf(0.0f, r.d.x, V.x), __builtin_fmaf(0.0f, r.d.y, V.y),
+                        __builtin_fmaf(0.0f, r.d.z, V.z));
+        else
+            r.d = rn::add(rn::mul(rn::sub(1.0f, w), r.d), rn::mul(w, V));
         const vec3 step = rn::mul(r.d, h);
         if (r.face < 0) {
             const float L2 = rn::dot(step, step);
@@ -581,6 +588,13 @@ __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, co
         }
     }
     return ray_finished(r, I);
+}
+
+template <class F>
+__device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, const SceneParams& S,
+                                            const float4* lds, int budget, QueryStats& st) {
+    if (I.field_weight == 1.0f) return ray_advance_impl<F, true>(r, I, S, lds, budget, st);
+    return ray_advance_impl<F, false>(r, I, S, lds, budget, st);
 }
 
 // Colour (alpha forced to 1, black when nothing was hit) and end point of a finished ray
