@@ -320,8 +320,12 @@ def main() -> int:
     value = steps_per_frame_set * a.steps / elapsed
     dom = max(per_field, key=lambda n: per_field[n]["raycast_ms"])
     d = per_field[dom]
-    kname = {"swirl": "FieldSwirl", "lens": "FieldLens", "hot_air": "FieldHotAir", "warp": "FieldWarp"}[dom]
-    roofline = {"bound": "valu", "kernel": f"void linon::raycast_kernel<lw::{kname}>(RaycastParams)",
+    # the kernel each field runs under the default math choice, as rocprofv3 names it
+    kname = {"swirl": "linon_wgslmath::raycast_kernel<lw::FieldSwirl>",
+             "lens": "linon::raycast_kernel<lw::FieldLens>",
+             "hot_air": "linon::raycast_kernel<lw::FieldHotAirT<true> >",
+             "warp": "linon_wgslmath::raycast_kernel<lw::FieldWarp>"}[dom]
+    roofline = {"bound": "valu", "kernel": f"void {kname}(RaycastParams)",
                 "note": "compute-bound on FP32 vector ALUs (no matrix cores); peak = MI355X FP32 157.3 TFLOP/s",
                 "achieved": d["tflops"],
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": d["tflops"] / PEAK_FP32_TFLOPS,

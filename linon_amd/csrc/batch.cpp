@@ -146,8 +146,9 @@ linon_status Context::batch_launch(BatchState& B, const linon_batch_tuning& T, h
         P.scene = S;
         // groups 1, 2, 3, 4, ... go to side streams 0, 1, 2, 0, ... (the first stays on s)
         hipStream_t gs = gi == 0 ? s : batch_side[(gi - 1) % (size_t)nside];
-        LB_CHECK(this, wgsl_math_for(g.field) ? linon_wgslmath::launch_batch(g.field, P, blocks, 64u * (unsigned)wpb, gs)
-                                               : launch_batch(g.field, P, blocks, 64u * (unsigned)wpb, gs));
+        const int kf = kernel_field(g.field);
+        LB_CHECK(this, wgsl_math_for(g.field) ? linon_wgslmath::launch_batch(kf, P, blocks, 64u * (unsigned)wpb, gs)
+                                               : launch_batch(kf, P, blocks, 64u * (unsigned)wpb, gs));
     }
     for (int k = 0; k < nside; ++k) {
         LB_CHECK(this, hipEventRecord(batch_join[k], batch_side[k]));

@@ -213,8 +213,8 @@ linon_status Context::render_with(const linon_integrator& ig, hipStream_t s) {
         LN_CHECK_HIP(this, hipModuleLaunchKernel(jit_fn, blocks, 1, 1, 64, 1, 1, (unsigned)shmem, s, args, nullptr));
     } else {
         LN_CHECK_HIP(this, wgsl_math_for(field_id)
-                               ? linon_wgslmath::launch_raycast_static(field_id, P, blocks, shmem, s)
-                               : launch_raycast_static(field_id, P, blocks, shmem, s));
+                               ? linon_wgslmath::launch_raycast_static(kernel_field(field_id), P, blocks, shmem, s)
+                               : launch_raycast_static(kernel_field(field_id), P, blocks, shmem, s));
     }
     LN_CHECK_HIP(this, hipEventRecord(ev[1], s));
     OverlayParams O;
@@ -237,12 +237,17 @@ linon_status Context::render_with(const linon_integrator& ig, hipStream_t s) {
 }
 
 bool Context::wgsl_math_for(int field) const { return wgsl_math_default(math, field); }
+int Context::kernel_field(int field) const { return kernel_field_id(math, field); }
 
 namespace {
 thread_local std::string g_unbound_error;
 }
 void set_unbound_error(const std::string& msg) { g_unbound_error = msg; }
 const std::string& unbound_error() { return g_unbound_error; }
+
+int kernel_field_id(int math, int field) {
+    return (math == LINON_MATH_AUTO && field == LINON_FIELD_HOT_AIR) ? LINON_KERNEL_HOT_AIR_FAITHFUL_ROOT : field;
+}
 
 bool wgsl_math_default(int math, int field) {
     if (math == LINON_MATH_IEEE) return false;

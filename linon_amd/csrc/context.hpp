@@ -113,6 +113,7 @@ struct Context {
     int field_id = -1;
     int math = LINON_MATH_AUTO;
     bool wgsl_math_for(int field) const;  // which kernel build a predefined field runs with
+    int kernel_field(int field) const;    // and which of its kernel-table entries
     bool jit_wgsl_math = false;           // the build the current WGSL body was compiled as
     hipModule_t jit_module = nullptr;
     hipFunction_t jit_fn = nullptr;
@@ -196,6 +197,9 @@ struct Context {
 void fill_step(StepParams& I, const linon_integrator& ig, float field_weight);
 // The kernel build (WGSL accuracy or IEEE) that linon_math value `math` selects for a field.
 bool wgsl_math_default(int math, int field);
+// The kernel-table id a predefined field runs with under a math choice: under AUTO hot air takes
+// its faithful-root variant (LINON_KERNEL_HOT_AIR_FAITHFUL_ROOT), every other field its own id.
+int kernel_field_id(int math, int field);
 // Error message of the calling thread's last failed entry point that takes no context
 // (linon_last_error(NULL) returns it).
 void set_unbound_error(const std::string& msg);

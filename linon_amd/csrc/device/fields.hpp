@@ -93,20 +93,34 @@ struct FieldLens {  // v - 0.3 * p / max(|p|^3, 1e-3)
         return v - div3_rn_lens(0.3f * p, max(r * r * r, 0.001f));  // 0.3 p / max(r^3, 0.001)
     }
 };
-struct FieldHotAir {  // refraction(T(p_prev), T(p), v, normalize(p - c)), T = 20 + 80 sigmoid(8 (0.5 - |p - c|))
+// Hot air: refraction(T(p_prev), T(p), v, normalize(p - c)), T = 20 + 80 sigmoid(8 (0.5 - |p - c|)).
+// FaithfulRoot: the distance |p - c| (the temperature's argument and the normal's length) takes
+// the hardware square root, within 1 ulp (WGSL bounds sqrt only to inverseSqrt's accuracy),
+// instead of the correctly rounded one: the build AUTO picks for hot air.  Its exit points are as
+// close to the FP64 oracle as the exact root's, and the step is 8% shorter (the correctly
+// rounded root takes 14 instructions).
+template <bool FaithfulRoot>
+struct FieldHotAirT {
+    __device__ __forceinline__ static vec3 offset(vec3 q) { return q - vec3f(0.0f, -0.5f, 0.0f); }
+    __device__ __forceinline__ static float distance(vec3 q) {
+        const vec3 d = offset(q);
+        return FaithfulRoot ? __builtin_amdgcn_sqrtf(dot(d, d)) : length(d);
+    }
     __device__ __forceinline__ static float temperature(vec3 q) {
-        const vec3 c = vec3f(0.0f, -0.5f, 0.0f);
-        return 20.0f + 80.0f * sigmoid_floor(8.0f * (0.5f - length(q - c)));  // in [20, 100]
+        return 20.0f + 80.0f * sigmoid_floor(8.0f * (0.5f - distance(q)));  // in [20, 100]
+    }
+    __device__ __forceinline__ static vec3 normal(vec3 p) {
+        return FaithfulRoot ? div3_rn_bounded(offset(p), distance(p)) : normalize(offset(p));
     }
     __device__ __forceinline__ vec3 operator()(vec3 p_prev, vec3 p, vec3, vec3 v, float) const {
-        const vec3 c = vec3f(0.0f, -0.5f, 0.0f);
-        return refraction_in_range(temperature(p_prev), temperature(p), v, normalize(p - c));
+        return refraction_in_range(temperature(p_prev), temperature(p), v, normal(p));
     }
     __device__ __forceinline__ vec3 first_stage(vec3 p, vec3, vec3 v, float) const {
-        const vec3 c = vec3f(0.0f, -0.5f, 0.0f);
-        return refraction_in_range_same(temperature(p), v, normalize(p - c));
+        return refraction_in_range_same(temperature(p), v, normal(p));
     }
 };
+using FieldHotAir = FieldHotAirT<false>;
+using FieldHotAirFaithfulRoot = FieldHotAirT<true>;
 struct FieldWarp {  // v0 + 0.3 * (sin 4p.y, 0, sin 4p.x)
     __device__ __forceinline__ vec3 operator()(vec3, vec3 p, vec3 v0, vec3, float) const {
         return v0 + 0.3f * vec3f(sin(4.0f * p.y), 0.0f, sin(4.0f * p.x));

@@ -10,6 +10,9 @@
 #define LINON_TILE 8          // 8x8 pixels per wave64, the reference's @workgroup_size(8, 8)
 #define LINON_NUM_XCD 8       // MI355X: 256 CUs in 8 accelerator complex dies
 #define LINON_MAX_BRUTE_LDS 96  // faces staged in LDS by the brute-force path
+// kernel-table id (not a public field id) of hot air with the faithful distance root, which
+// the math choice AUTO runs hot air with (fields.hpp FieldHotAirT)
+#define LINON_KERNEL_HOT_AIR_FAITHFUL_ROOT 21
 
 // counters[] slots
 #define LINON_CNT_STEPS 0

@@ -50,6 +50,8 @@ bool field_is_predefined(int id) { return (id >= 1 && id <= 9) || (id >= 16 && i
         case 18: return (const void*)&KERNEL<lw::FieldLens>;           \
         case 19: return (const void*)&KERNEL<lw::FieldHotAir>;         \
         case 20: return (const void*)&KERNEL<lw::FieldWarp>;           \
+        case LINON_KERNEL_HOT_AIR_FAITHFUL_ROOT:                       \
+            return (const void*)&KERNEL<lw::FieldHotAirFaithfulRoot>;  \
         default: return nullptr;                                       \
     }
 
@@ -78,6 +80,7 @@ const void* raycast_kernel_for(int id) {
         case 18: return (const void*)&raycast_kernel<lw::FieldLens>;
         case 19: return (const void*)&raycast_kernel<lw::FieldHotAir>;
         case 20: return (const void*)&raycast_kernel<lw::FieldWarp>;
+        case LINON_KERNEL_HOT_AIR_FAITHFUL_ROOT: return (const void*)&raycast_kernel<lw::FieldHotAirFaithfulRoot>;
         default: return nullptr;
     }
 }
@@ -272,6 +275,7 @@ __global__ void __launch_bounds__(256) div_selftest_refraction_kernel(unsigned l
     // the refracting fields' first RK4 stage, f.first_stage(p, v0, v, t) = f(p, p, v0, v, t),
     // with p taken from the random normals (components in [-1.5, 1.5])
     bad += first_stage_mismatches(lw::FieldHotAir{}, nrm, v);
+    bad += first_stage_mismatches(lw::FieldHotAirFaithfulRoot{}, nrm, v);
     bad += first_stage_mismatches(lw::FieldMirageSpherical{}, nrm, v);
     bad += first_stage_mismatches(lw::FieldMiragePlane{}, nrm, v);
     bad += first_stage_mismatches(lw::FieldMirageSphericalSigmoid{}, nrm, v);
@@ -384,7 +388,7 @@ hipError_t run_division_selftest(unsigned long long random_pairs, unsigned long 
             const unsigned long long m = random_pairs - off < chunk ? random_pairs - off : chunk;
             hipLaunchKernelGGL(div_selftest_refraction_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
                                nullptr, 0x5851F42D4C957F2Dull + off, m, d + 1);
-            n += 21 * m;
+            n += 24 * m;
         }
         for (unsigned long long off = 0; off < random_pairs && e == hipSuccess; off += chunk) {
             const unsigned long long m = random_pairs - off < chunk ? random_pairs - off : chunk;

@@ -90,8 +90,9 @@ extern "C" linon_status linon_flowmap_noalloc(int device, int field, int math, c
     e = hipMemcpy(d_in, states, bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = linon::sg_fill_i32(d_steps, n, 0, nullptr);
     if (e == hipSuccess)
-        e = linon::wgsl_math_default(math, field) ? linon_wgslmath::sg_flowmap(field, F, nullptr)
-                                                   : linon::sg_flowmap(field, F, nullptr);
+        e = linon::wgsl_math_default(math, field)
+                ? linon_wgslmath::sg_flowmap(linon::kernel_field_id(math, field), F, nullptr)
+                : linon::sg_flowmap(linon::kernel_field_id(math, field), F, nullptr);
     if (e == hipSuccess) e = hipMemcpy(exits, d_out, bytes, hipMemcpyDeviceToHost);
     if (e == hipSuccess && steps) e = hipMemcpy(steps, d_steps, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return fail(LINON_ERR_HIP, std::string("flowmap_noalloc: ") + hipGetErrorString(e));

@@ -268,8 +268,8 @@ linon_status Context::surrogate_train(const linon_surrogate_config& cfg, linon_s
     F.states = S.d_states;
     F.n = N;
     F.exits = S.d_targets;
-    LS_CHECK(this, wgsl_math_for(cfg.field) ? linon_wgslmath::sg_flowmap(cfg.field, F, s)
-                                              : sg_flowmap(cfg.field, F, s));
+    LS_CHECK(this, wgsl_math_for(cfg.field) ? linon_wgslmath::sg_flowmap(kernel_field(cfg.field), F, s)
+                                              : sg_flowmap(kernel_field(cfg.field), F, s));
     // the coarse flow map of the same states, and the network targets: (exit - coarse) * scale,
     // each scale making its residual's RMS 1/4; without a coarse map, exit / R and the direction
     float* base = nullptr;
@@ -278,8 +278,8 @@ linon_status Context::surrogate_train(const linon_surrogate_config& cfg, linon_s
         ls::FlowmapParams Fc = F;
         Fc.step = surrogate_coarse_step_params();
         Fc.exits = base;
-        LS_CHECK(this, wgsl_math_for(cfg.field) ? linon_wgslmath::sg_flowmap(cfg.field, Fc, s)
-                                                  : sg_flowmap(cfg.field, Fc, s));
+        LS_CHECK(this, wgsl_math_for(cfg.field) ? linon_wgslmath::sg_flowmap(kernel_field(cfg.field), Fc, s)
+                                                  : sg_flowmap(kernel_field(cfg.field), Fc, s));
         float* acc = nullptr;
         LS_CHECK(this, dalloc(acc, 2));
         LS_CHECK(this, hipMemsetAsync(acc, 0, 8, s));
@@ -552,8 +552,9 @@ linon_status Context::surrogate_frame(hipStream_t s) {
     const bool q_is_v0 = S.cfg.field == LINON_FIELD_SWIRL || S.cfg.field == LINON_FIELD_WARP ||
                          (S.cfg.field >= 5 && S.cfg.field <= 7);
     const bool wgsl = wgsl_math_for(S.cfg.field);
+    const int kf = kernel_field(S.cfg.field);
     auto flowmap = [&](const ls::FlowmapParams& F) {
-        return wgsl ? linon_wgslmath::sg_flowmap(S.cfg.field, F, s) : sg_flowmap(S.cfg.field, F, s);
+        return wgsl ? linon_wgslmath::sg_flowmap(kf, F, s) : sg_flowmap(kf, F, s);
     };
     LS_CHECK(this, mark(0));
     LS_CHECK(this, sg_fill_i32(S.f_count, 4, 0, s));
@@ -569,7 +570,7 @@ linon_status Context::surrogate_frame(hipStream_t s) {
     A.mapping = d_mapping;
     A.steps = d_steps;
     A.q_is_v0 = q_is_v0 ? 1 : 0;
-    LS_CHECK(this, wgsl ? linon_wgslmath::sg_approach(S.cfg.field, A, s) : sg_approach(S.cfg.field, A, s));
+    LS_CHECK(this, wgsl ? linon_wgslmath::sg_approach(kf, A, s) : sg_approach(kf, A, s));
     LS_CHECK(this, sg_compact_entered(S.f_entered, S.f_entry, npx, S.f_states, S.f_texel, n_in, s));
     LS_CHECK(this, mark(1));
     // 2. coarse flow map (multi-fidelity form), then the MLP in one fused launch
@@ -903,8 +904,9 @@ extern "C" linon_status linon_flowmap(int device, int field, int math, const lin
         F.n = n;
         F.exits = d_out;
         F.steps = d_steps;
-        if (ok(linon::wgsl_math_default(math, field) ? linon_wgslmath::sg_flowmap(field, F, nullptr)
-                                                      : linon::sg_flowmap(field, F, nullptr)) &&
+        const int kf = linon::kernel_field_id(math, field);
+        if (ok(linon::wgsl_math_default(math, field) ? linon_wgslmath::sg_flowmap(kf, F, nullptr)
+                                                      : linon::sg_flowmap(kf, F, nullptr)) &&
             ok(hipDeviceSynchronize()) && ok(hipMemcpy(exits, d_out, bytes, hipMemcpyDeviceToHost)) && steps)
             ok(hipMemcpy(steps, d_steps, (size_t)n * 4, hipMemcpyDeviceToHost));
     }
