@@ -96,9 +96,10 @@ struct FieldLens {  // v - 0.3 * p / max(|p|^3, 1e-3)
 // Hot air: refraction(T(p_prev), T(p), v, normalize(p - c)), T = 20 + 80 sigmoid(8 (0.5 - |p - c|)).
 // FaithfulRoot: the distance |p - c| (the temperature's argument and the normal's length) takes
 // the hardware square root, within 1 ulp (WGSL bounds sqrt only to inverseSqrt's accuracy),
-// instead of the correctly rounded one: the build AUTO picks for hot air.  Its exit points are as
-// close to the FP64 oracle as the exact root's, and the step is 8% shorter (the correctly
-// rounded root takes 14 instructions).
+// instead of the correctly rounded one, and the sigmoid's exponential exp_faithful (within
+// about 1.5 ulp) instead of the full-precision library's: the variant AUTO picks for hot air.
+// Its exit points are as close to the FP64 oracle as the exact variant's, and the step is 11%
+// shorter (the correctly rounded root takes 14 instructions, the library exponential 13).
 template <bool FaithfulRoot>
 struct FieldHotAirT {
     __device__ __forceinline__ static vec3 offset(vec3 q) { return q - vec3f(0.0f, -0.5f, 0.0f); }
@@ -107,7 +108,12 @@ struct FieldHotAirT {
         return FaithfulRoot ? __builtin_amdgcn_sqrtf(dot(d, d)) : length(d);
     }
     __device__ __forceinline__ static float temperature(vec3 q) {
-        return 20.0f + 80.0f * sigmoid_floor(8.0f * (0.5f - distance(q)));  // in [20, 100]
+        const float x = 8.0f * (0.5f - distance(q));
+#ifdef LINON_FAST_MATH
+        return 20.0f + 80.0f * sigmoid_floor(x);  // in [20, 100]
+#else
+        return 20.0f + 80.0f * (FaithfulRoot ? sigmoid_floor_faithful(x) : sigmoid_floor(x));
+#endif
     }
     __device__ __forceinline__ static vec3 normal(vec3 p) {
         return FaithfulRoot ? div3_rn_bounded(offset(p), distance(p)) : normalize(offset(p));
