@@ -240,6 +240,7 @@ SIGNATURES = {
     "linon_set_output_srgb": (C.c_int, [_P, C.c_int]),
     "linon_measure_fma_rate": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
     "linon_selftest_sqrt": (C.c_int, [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "linon_selftest_exp": (C.c_int, [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_float)]),
     "linon_set_field_predefined": (C.c_int, [_P, C.c_int]),
     "linon_set_field_wgsl": (C.c_int, [_P, C.c_char_p]),
     "linon_wgsl_translate": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
@@ -636,6 +637,17 @@ def selftest_sqrt(device: int = 0) -> dict:
     if st != LINON_OK:
         raise LinonError(st, "linon_selftest_sqrt failed")
     return {"mismatches": m.value, "comparisons": n.value}
+
+
+def selftest_exp(device: int = 0) -> dict:
+    """The faithful exponential of hot air's default variant at every float in [-87, 87] against
+    the FP64 exponential (on the GPU): results off by more than 2 ulp, the largest error in ulp."""
+    lib = load_library()
+    o, n, m = C.c_uint64(0), C.c_uint64(0), C.c_float(0.0)
+    st = lib.linon_selftest_exp(device, C.byref(o), C.byref(n), C.byref(m))
+    if st != LINON_OK:
+        raise LinonError(st, "linon_selftest_exp failed")
+    return {"over_2ulp": o.value, "comparisons": n.value, "max_ulp": m.value}
 
 
 def measure_fma_rate(device: int = 0) -> float:

@@ -3,6 +3,8 @@
 // so that a caller may capture linon_render into a HIP graph).
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "device/batch.hpp"
 #include "device/fields.hpp"
 #include "device/raycast.hpp"
@@ -112,7 +114,8 @@ hipError_t launch_clear_u64(unsigned long long* p, int n, hipStream_t s) {
 }
 
 // ---- IEEE build only: self-tests of its arithmetic (the reciprocal-based quotients against
-// IEEE division, sqrt_rn against the compiler's square root) and the FMA-rate measurement ----
+// IEEE division, sqrt_rn against the compiler's square root, exp_faithful against the FP64
+// exponential) and the FMA-rate measurement ----
 #ifndef LINON_FAST_MATH
 __device__ __forceinline__ unsigned fbits(float x) { return __float_as_uint(x); }
 
@@ -235,6 +238,26 @@ __global__ void __launch_bounds__(256) div_selftest_random_kernel(unsigned long 
     if (bad) atomicAdd(mism, (unsigned long long)bad);
 }
 
+// every float y in [-87, 87] whose bit pattern has the top byte `hi`: lw::exp_faithful(y)
+// against the FP64 exponential rounded to float, in units of the float spacing at the result
+// (ulp = 2^(exponent - 23), subnormal spacing below 2^-126); counts results off by more than
+// 2 ulp and keeps the largest error (in 1/1024 ulp, as an integer for atomicMax)
+__global__ void __launch_bounds__(256) exp_selftest_kernel(unsigned hi, unsigned long long* over,
+                                                           unsigned* max_milli_ulp) {
+    const unsigned lo = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lo >= (1u << 24)) return;
+    const float y = __uint_as_float((hi << 24) | lo);
+    if (!(__builtin_fabsf(y) <= 87.0f)) return;
+    const double ref = exp((double)y);
+    const float got = lw::exp_faithful(y);
+    int e;
+    (void)frexp(ref, &e);  // ref = m 2^e, m in [0.5, 1)
+    const double ulp = ldexp(1.0, (e - 1 < -126 ? -126 : e - 1) - 23);
+    const double err = fabs((double)got - ref) / ulp;
+    if (err > 2.0) atomicAdd(over, 1ull);
+    atomicMax(max_milli_ulp, (unsigned)fmin(err * 1024.0, 4.0e9));
+}
+
 // every float x whose bit pattern has the top byte `hi`: sqrt_rn(x) against the compiler's
 // correctly rounded square root
 __global__ void __launch_bounds__(256) sqrt_selftest_kernel(unsigned hi, unsigned long long* mism) {
@@ -339,6 +362,30 @@ hipError_t run_sqrt_selftest(unsigned long long* mism, unsigned long long* cases
     (void)hipFree(d);
     *mism = h;
     *cases = 1ull << 32;
+    return e;
+}
+
+hipError_t run_exp_selftest(unsigned long long* over, unsigned long long* cases, float* max_ulp) {
+    unsigned long long* d = nullptr;
+    hipError_t e = hipMalloc(&d, 2 * sizeof(unsigned long long));
+    if (e != hipSuccess) return e;
+    e = hipMemset(d, 0, 2 * sizeof(unsigned long long));
+    unsigned* dm = reinterpret_cast<unsigned*>(d + 1);
+    unsigned long long n = 0;
+    for (unsigned hi = 0; hi < 256 && e == hipSuccess; ++hi) {
+        hipLaunchKernelGGL(exp_selftest_kernel, dim3((1u << 24) / 256), dim3(256), 0, nullptr, hi, d, dm);
+        n += 1ull << 24;
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    unsigned long long h[2] = {0, 0};
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    *over = h[0];
+    *cases = n;
+    unsigned mm = 0;
+    std::memcpy(&mm, &h[1], sizeof(mm));
+    *max_ulp = (float)mm / 1024.0f;
     return e;
 }
 

@@ -32,6 +32,7 @@ LINON_DECLARE_KERNEL_LAUNCHERS
 hipError_t run_fma_rate(int cus, double* tflops);
 // lw::sqrt_rn at every float against the compiler's square root (IEEE build)
 hipError_t run_sqrt_selftest(unsigned long long* mism, unsigned long long* cases);
+hipError_t run_exp_selftest(unsigned long long* over, unsigned long long* cases, float* max_ulp);
 hipError_t run_division_selftest(unsigned long long random_pairs, unsigned long long* recip_mism,
                                  unsigned long long* quot_mism, unsigned long long* cases);
 }  // namespace linon

@@ -370,6 +370,16 @@ def test_square_root_is_ieee():
     assert res["mismatches"] == 0, res
 
 
+def test_faithful_exponential_is_within_two_ulp():
+    """Hot air's default variant takes the sigmoid's exponential from a split product and the
+    hardware 2^t: at every float in [-87, 87] within 2 ulp of the FP64 exponential."""
+    print("[case] exp self-test", flush=True)
+    res = L.selftest_exp()
+    print(res, flush=True)
+    assert res["comparisons"] == 1 << 32
+    assert res["over_2ulp"] == 0 and res["max_ulp"] <= 2.0, res
+
+
 def test_wgsl_integer_division_is_total(view):
     """A WGSL body's integer / and % follow WGSL, not C++: x / 0 = x, x % 0 = 0,
     INT_MIN / -1 = INT_MIN, INT_MIN % -1 = 0 (i32 and u32).  The body returns a constant
