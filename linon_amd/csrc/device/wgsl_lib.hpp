@@ -159,8 +159,26 @@ __device__ __forceinline__ float quot_rn_signed_zero(float a, float b, float y) 
 // and the FMA forms it exactly even where it is subnormal; the corrected quotient is then the
 // correctly rounded one, as in the normal range.  For s in (2^22, 2^60] the bound stays 2^-60
 // (div3_rn_bounded's); anything else takes the ordinary division.  The test orders magnitudes
-// by their bit patterns doubled minus 2, which sends both zeros to the top.  Same result as
-// v / s for every finite p.
+// by their bit patterns doubled minus 2, which sends both zeros to the top.  The captured rays'
+// smaller components take quot_lens_floor where s is the floor 0.001 itself, and the ordinary
+// division elsewhere.  Same result as v / s for every finite p.
+//
+// quot_lens_floor(a) = a / c correctly rounded, c = RN(0.001) = 0x3a83126f, for |a| <= 2^12 (the
+// lens's components while s == c: r^3 <= c gives |0.3 p_i| <= 0.0301).  With a = j 2^-149:
+//  * |j| <= 8388: a / c = (1000 - 4.7497e-5) j 2^-149 is below 2^-126, so it rounds to the
+//    subnormal grid 2^-149, and 4.7497e-5 |j| <= 0.3985 < 1/2 makes the nearest point 1000 j;
+//    1000 a (below 2^23 units of 2^-149, and +-0 for a = +-0) is that point, exactly.
+//  * otherwise a / c >= 2^-126 is normal: a 2^48 (exact) is at least 2^-88 and at most 2^60, in
+//    quot_rn_signed_zero's range with s = c, so RN(a 2^48 / c) is correctly rounded, and its
+//    product with 2^-48 stays normal and exact: RN(a / c).
+// Y_C = RN(1 / c) = 0x4479ffff, the reciprocal recip_rn(c) forms.
+__device__ __forceinline__ float quot_lens_floor(float a) {
+#pragma clang fp contract(off)
+    constexpr float C = 0.001f, Y_C = 999.99993896484375f;
+    const float small = a * 1000.0f;
+    const float big = quot_rn_signed_zero(a * 0x1p48f, C, Y_C) * 0x1p-48f;
+    return __builtin_fabsf(a) <= 8388.0f * 0x1p-149f ? small : big;
+}
 __device__ __forceinline__ vec3 div3_rn_lens(vec3 v, float s) {
 #ifdef LINON_FAST_MATH
     return v / s;
@@ -175,6 +193,7 @@ __device__ __forceinline__ vec3 div3_rn_lens(vec3 v, float s) {
         const float y = recip_rn(s);
         return {quot_rn_signed_zero(v.x, s, y), quot_rn_signed_zero(v.y, s, y), quot_rn_signed_zero(v.z, s, y)};
     }
+    if (s == 0.001f) return {quot_lens_floor(v.x), quot_lens_floor(v.y), quot_lens_floor(v.z)};
     return {v.x / s, v.y / s, v.z / s};
 #endif
 }

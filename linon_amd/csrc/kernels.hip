@@ -193,6 +193,22 @@ __global__ void __launch_bounds__(256) div_selftest_bounded_kernel(int ea, float
     if (bad) atomicAdd(mism, (unsigned long long)bad);
 }
 
+// the lens quotient at its floor s = 0.001 for every numerator of magnitude below 2^13 (both
+// signs, subnormals included): the captured rays' components
+__global__ void __launch_bounds__(256) div_selftest_lens_floor_kernel(int biased_exp, unsigned long long* mism) {
+    const unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (1u << 23)) return;
+    const float a = __uint_as_float(((unsigned)biased_exp << 23) | m);
+    const float s = 0.001f;
+    const lw::vec3 q = lw::div3_rn_lens(lw::vec3f(a, -a, 0.0f), s);
+    unsigned bad = 0;
+    bad += fbits(q.x) != fbits(a / s);
+    bad += fbits(q.y) != fbits(-a / s);
+    bad += fbits(q.z) != fbits(0.0f / s);
+    bad += fbits(lw::quot_lens_floor(a)) != fbits(a / s);
+    if (bad) atomicAdd(mism, (unsigned long long)bad);
+}
+
 // random operands (SplitMix64 stream): sign, exponent in [-70, 70] (so the out-of-range and
 // zero paths are taken too), mantissa uniform
 __device__ __forceinline__ unsigned long long smix(unsigned long long z) {
@@ -425,6 +441,10 @@ hipError_t run_division_selftest(unsigned long long random_pairs, unsigned long 
             hipLaunchKernelGGL(div_selftest_bounded_kernel, dim3(mblocks), dim3(256), 0, nullptr, ea, b, d + 1);
             n += 12ull << 23;
         }
+    for (int be = 0; be < 127 + 13 && e == hipSuccess; ++be) {  // lens floor: |a| < 2^13
+        hipLaunchKernelGGL(div_selftest_lens_floor_kernel, dim3(mblocks), dim3(256), 0, nullptr, be, d + 1);
+        n += 4ull << 23;
+    }
     for (unsigned hi = 0; hi < 256 && e == hipSuccess; ++hi) {  // sigmoid_floor at every float
         hipLaunchKernelGGL(div_selftest_sigmoid_kernel, dim3((1u << 24) / 256), dim3(256), 0, nullptr, hi, d + 1);
         n += 3ull << 24;
