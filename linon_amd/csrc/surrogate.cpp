@@ -571,7 +571,7 @@ linon_status Context::surrogate_frame(hipStream_t s) {
     A.steps = d_steps;
     A.q_is_v0 = q_is_v0 ? 1 : 0;
     LS_CHECK(this, wgsl ? linon_wgslmath::sg_approach(kf, A, s) : sg_approach(kf, A, s));
-    LS_CHECK(this, sg_compact_entered(S.f_entered, S.f_entry, npx, S.f_states, S.f_texel, n_in, s));
+    LS_CHECK(this, sg_compact_entered(S.f_entered, S.f_entry, W, H, S.f_states, S.f_texel, n_in, s));
     LS_CHECK(this, mark(1));
     // 2. coarse flow map (multi-fidelity form), then the MLP in one fused launch
     const bool coarse = S.cfg.coarse_h > 0.0f;

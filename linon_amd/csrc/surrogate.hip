@@ -304,16 +304,24 @@ __global__ void __launch_bounds__(256) accept_kernel(const float* out, const flo
     }
 }
 
-// Compacts the rays that entered the sphere: entry state rows in ray order of a prefix of
-// texels; texel_of[i] remembers where ray i's result goes.
-__global__ void __launch_bounds__(256) compact_entered_kernel(const int* entered, const float* entry_states, int n,
-                                                              float* states, int* texel_of, int* count) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool e = t < n && entered[t];
+// Compacts the rays that entered the sphere into a prefix of entry-state rows; texel_of[i]
+// remembers where ray i's result goes.  Threads walk the image in the approach kernel's 8x8
+// tile order, one wave per tile, so each wave appends one tile's entering rays together: the
+// RK4 legs that run over the compacted rows (coarse flow map, fallback, re-judging) then see
+// neighbouring pixels side by side in a wave, as the plain kernel's tiles do, rather than 64
+// pixels of one image row.
+__global__ void __launch_bounds__(256) compact_entered_kernel(const int* entered, const float* entry_states, int W,
+                                                              int H, float* states, int* texel_of, int* count) {
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int tiles_x = (W + LINON_TILE - 1) / LINON_TILE;
+    const int gx = (tile % tiles_x) * LINON_TILE + (lane & (LINON_TILE - 1));
+    const int gy = (tile / tiles_x) * LINON_TILE + (lane / LINON_TILE);
+    const int t = (H - 1 - gy) * W + gx;  // the approach kernel's texel of (gx, gy)
+    const bool e = gx < W && gy < H && entered[t];
     // wave-aggregated append: one atomic per wave, lanes placed by their rank among the
     // entering lanes of the wave
     const unsigned long long mask = __ballot(e);
-    const int lane = threadIdx.x & 63;
     int base = 0;
     if (lane == 0 && mask) base = atomicAdd(count, __popcll(mask));
     base = __shfl(base, 0, 64);
@@ -449,9 +457,11 @@ hipError_t sg_accept(const float* out, const float* base, const int* texel_of, i
                        pred_exit, fallback, n_fallback, accepted);
     return hipGetLastError();
 }
-hipError_t sg_compact_entered(const int* entered, const float* entry_states, int n, float* states, int* texel_of,
-                              int* count, hipStream_t s) {
-    hipLaunchKernelGGL(compact_entered_kernel, LS_GRID(n), 0, s, entered, entry_states, n, states, texel_of, count);
+hipError_t sg_compact_entered(const int* entered, const float* entry_states, int W, int H, float* states,
+                              int* texel_of, int* count, hipStream_t s) {
+    const unsigned tiles = (unsigned)(((W + LINON_TILE - 1) / LINON_TILE) * ((H + LINON_TILE - 1) / LINON_TILE));
+    hipLaunchKernelGGL(compact_entered_kernel, dim3((tiles + 3) / 4), dim3(256), 0, s, entered, entry_states, W, H,
+                       states, texel_of, count);
     return hipGetLastError();
 }
 hipError_t sg_apply_rk4(const int* list, int n, const int* n_dev, const float* exits, const int* texel_of,

@@ -36,8 +36,8 @@
     hipError_t sg_accept(const float* out, const float* base, const int* texel_of, int n, const int* n_dev, \
                          float scale_p, float thr, float* mapping, float* pred_exit, int* fallback, int* n_fallback, \
                          int* accepted, hipStream_t s); \
-    hipError_t sg_compact_entered(const int* entered, const float* entry_states, int n, float* states, int* texel_of, \
-                                  int* count, hipStream_t s); \
+    hipError_t sg_compact_entered(const int* entered, const float* entry_states, int W, int H, float* states, \
+                                  int* texel_of, int* count, hipStream_t s); \
     hipError_t sg_apply_rk4(const int* list, int n, const int* n_dev, const float* exits, const int* texel_of, \
                             const float* pred_exit, float* mapping, float* err_out, int write_mapping, hipStream_t s); \
     hipError_t sg_accepted_mask(const int* accepted, const int* texel_of, int n, const int* n_dev, int npx, \
