@@ -371,8 +371,11 @@ linon_status linon_flowmap_noalloc(int device, int field, int math, const linon_
  * temperature from it; lw::refraction_in_range against lw::refraction on random_pairs random
  * temperatures in its range, directions and normals, and on the same draws the one-temperature
  * form lw::refraction_in_range_same and the refracting fields' first RK4 stage (hot air and the
- * four mirages: first_stage(p, ...) against the field at p_prev = p).  quotient_mismatches
- * counts all but the reciprocal's.  Writes the mismatch counts and the number of
+ * four mirages: first_stage(p, ...) against the field at p_prev = p); the bounded divisions of
+ * normalize and the lens field (lw::div3_rn_bounded, lw::div3_rn_lens) at every numerator
+ * mantissa of 18 exponents from the subnormals up against 19 denominators, and the lens
+ * quotient at its floor s = 0.001 (lw::quot_lens_floor) at every numerator below 2^13 in
+ * magnitude.  quotient_mismatches counts all but the reciprocal's.  Writes the mismatch counts and the number of
  * comparisons. */
 linon_status linon_selftest_division(int device, uint64_t random_pairs, uint64_t* recip_mismatches,
                                      uint64_t* quotient_mismatches, uint64_t* comparisons);
