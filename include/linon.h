@@ -381,7 +381,8 @@ linon_status linon_selftest_division(int device, uint64_t random_pairs, uint64_t
                                      uint64_t* quotient_mismatches, uint64_t* comparisons);
 /* Diagnostics: the IEEE build's square root (lw::sqrt_rn: the hardware root plus a one-ulp
  * correction) against the compiler's correctly rounded square root at every one of the 2^32
- * floats, bit for bit (any NaN matches any NaN). */
+ * floats, bit for bit (any NaN matches any NaN), and its form without the rescaling of tiny
+ * operands (lw::sqrt_rn_no_tiny) at every float whose magnitude is not in (0, 2^-96). */
 linon_status linon_selftest_sqrt(int device, uint64_t* mismatches, uint64_t* comparisons);
 /* Diagnostics (no counterpart in the reference): the exponential of hot air's default variant
  * (lw::exp_faithful) at every float in [-87, 87] against the FP64 exponential, in units of the

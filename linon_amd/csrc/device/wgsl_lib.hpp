@@ -110,6 +110,24 @@ __device__ __forceinline__ float sqrt_rn(float x) {
     return tiny ? t * 0x1p-16f : t;
 #endif
 }
+// sqrt_rn for callers whose operand is never of magnitude in (0, 2^-96): +-0, at least 2^-96 in
+// magnitude, infinite or NaN (refraction's k, whose root is taken only for k >= 0).  Without the
+// rescaling of tiny inputs; the same result as sqrt_rn there.  (A negative subnormal would give
+// the hardware root's -0 instead of NaN.)
+__device__ __forceinline__ float sqrt_rn_no_tiny(float x) {
+#ifdef LINON_FAST_MATH
+    return __builtin_amdgcn_sqrtf(x);
+#else
+#pragma clang fp contract(off)
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float dn = __uint_as_float(__float_as_uint(s) - 1u);
+    const float up = __uint_as_float(__float_as_uint(s) + 1u);
+    const float r_dn = __builtin_fmaf(-dn, s, x);
+    const float r_up = __builtin_fmaf(-up, s, x);
+    float t = r_dn <= 0.0f ? dn : s;
+    return r_up > 0.0f ? up : t;
+#endif
+}
 
 __device__ __forceinline__ float dot(vec3 a, vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ vec3 cross(vec3 a, vec3 b) {
@@ -357,9 +375,14 @@ __device__ __forceinline__ vec3 refraction_impl(float t_in, float t_out, vec3 v_
         n_ref = -n;
     }
     float eta = quot(eta_in, eta_out);
-    float k = 1.0f - eta * eta * (1.0f - cosi * cosi);
+    // k = RN(1 - e t), e = RN(eta eta), t = RN(1 - cosi cosi) in [0, 1] (the FMA form the
+    // contracted expression 1 - eta * eta * (1 - cosi * cosi) compiles to).  k is never in
+    // (0, 2^-48): where |1 - e t| < 2^-48 the exact product e t lies in [0.5, 2], so the
+    // exponents of e and t add up to at least -2 and e t is a multiple of 2^-48, as is 1 - e t.
+    // The root therefore needs no rescaling of tiny operands.
+    const float k = __builtin_fmaf(-(eta * eta), __builtin_fmaf(-cosi, cosi, 1.0f), 1.0f);
     if (k < 0.0f) return reflect(v_in, n_ref);  // total internal reflection
-    return eta * v_in + (eta * cosi - sqrt_rn(k)) * n_ref;
+    return eta * v_in + (eta * cosi - sqrt_rn_no_tiny(k)) * n_ref;
 }
 __device__ __forceinline__ vec3 refraction(float t_in, float t_out, vec3 v_in, vec3 n) {
     return refraction_impl<false>(t_in, t_out, v_in, n);
@@ -390,9 +413,11 @@ __device__ __forceinline__ vec3 refraction_in_range_same(float t, vec3 v_in, vec
     } else {
         n_ref = -n;
     }
+    // 1 - t with t = RN(1 - cosi cosi) in [0, 1]: zero, or at least 2^-24 (t >= 1/2 is a
+    // multiple of 2^-24) -- no tiny operand for the root
     const float k = 1.0f - __builtin_fmaf(-cosi, cosi, 1.0f);
     if (k < 0.0f) return reflect(v_in, n_ref);
-    const float x = cosi - sqrt_rn(k);
+    const float x = cosi - sqrt_rn_no_tiny(k);
     auto axpy = [](float a, float b, float c) {
 #pragma clang fp contract(off)
         return a * b + c;

@@ -274,13 +274,17 @@ __global__ void __launch_bounds__(256) exp_selftest_kernel(unsigned hi, unsigned
     atomicMax(max_milli_ulp, (unsigned)fmin(err * 1024.0, 4.0e9));
 }
 
-// every float x whose bit pattern has the top byte `hi`: sqrt_rn(x) against the compiler's
-// correctly rounded square root
+// every float x whose bit pattern has the top byte `hi`: sqrt_rn(x), and sqrt_rn_no_tiny(x)
+// where |x| is not in (0, 2^-96), against the compiler's correctly rounded square root
 __global__ void __launch_bounds__(256) sqrt_selftest_kernel(unsigned hi, unsigned long long* mism) {
     const unsigned lo = blockIdx.x * blockDim.x + threadIdx.x;
     if (lo >= (1u << 24)) return;
     const float x = __uint_as_float((hi << 24) | lo);
-    if (!same_or_nan(lw::sqrt_rn(x), __builtin_sqrtf(x))) atomicAdd(mism, 1ull);
+    unsigned bad = !same_or_nan(lw::sqrt_rn(x), __builtin_sqrtf(x));
+    // the form without rescaling, on its domain: every float but those of magnitude in (0, 2^-96)
+    const float ax = __builtin_fabsf(x);
+    if (!(ax > 0.0f && ax < 0x1p-96f)) bad += !same_or_nan(lw::sqrt_rn_no_tiny(x), __builtin_sqrtf(x));
+    if (bad) atomicAdd(mism, (unsigned long long)bad);
 }
 
 // refraction_in_range against refraction: temperatures uniform in [-250, 1e4] (one in 64 at an
@@ -319,6 +323,14 @@ __global__ void __launch_bounds__(256) div_selftest_refraction_kernel(unsigned l
     bad += first_stage_mismatches(lw::FieldMiragePlane{}, nrm, v);
     bad += first_stage_mismatches(lw::FieldMirageSphericalSigmoid{}, nrm, v);
     bad += first_stage_mismatches(lw::FieldMiragePlaneSigmoid{}, nrm, v);
+    // refraction's k = RN(1 - e t) where e t is near 1 (e in [0.5, 2), t within 8 ulps of 1 / e,
+    // at most 1): never in (0, 2^-48), and its root without rescaling equals sqrt_rn's
+    const float e2 = 0.5f + 1.5f * rand_unit(r[2] ^ r[5]);
+    const int off = (int)(r[6] >> 60) - 8;
+    const float tt = __builtin_fminf(__uint_as_float(__float_as_uint(1.0f / e2) + off), 1.0f);
+    const float kk = __builtin_fmaf(-e2, tt, 1.0f);
+    bad += kk != 0.0f && __builtin_fabsf(kk) < 0x1p-48f;
+    if (kk >= 0.0f) bad += !same_or_nan(lw::sqrt_rn_no_tiny(kk), lw::sqrt_rn(kk));
     if (bad) atomicAdd(mism, (unsigned long long)bad);
 }
 #endif
@@ -455,7 +467,7 @@ hipError_t run_division_selftest(unsigned long long random_pairs, unsigned long 
             const unsigned long long m = random_pairs - off < chunk ? random_pairs - off : chunk;
             hipLaunchKernelGGL(div_selftest_refraction_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
                                nullptr, 0x5851F42D4C957F2Dull + off, m, d + 1);
-            n += 24 * m;
+            n += 26 * m;
         }
         for (unsigned long long off = 0; off < random_pairs && e == hipSuccess; off += chunk) {
             const unsigned long long m = random_pairs - off < chunk ? random_pairs - off : chunk;

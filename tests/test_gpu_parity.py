@@ -363,7 +363,9 @@ def test_shared_denominator_division_is_ieee():
 
 def test_square_root_is_ieee():
     """The IEEE build's square root (the hardware root plus a one-ulp correction, inputs below
-    2^-96 rescaled) equals the compiler's correctly rounded square root at every float."""
+    2^-96 rescaled) equals the compiler's correctly rounded square root at every float, and so
+    does its form without the rescaling (refraction's root) at every float whose magnitude is
+    not in (0, 2^-96)."""
     print("[case] sqrt self-test", flush=True)
     res = L.selftest_sqrt()
     print(res, flush=True)
