@@ -88,8 +88,12 @@ struct FieldSwirl {  // rotateY(v0, 0.5 * length(p))
     }
 };
 struct FieldLens {  // v - 0.3 * p / max(|p|^3, 1e-3)
+    // |p| enters only through s = max(|p|^3, 0.001).  Where dot(p, p) < 2^-96 the root is below
+    // 2^-47 whatever its last bits (or NaN, which max also drops), its cube below the floor, and
+    // s the floor 0.001 either way; elsewhere sqrt_rn_no_tiny is sqrt_rn.  So |p| needs no
+    // rescaling of tiny operands, and s (hence the field) is bit for bit length()'s.
     __device__ __forceinline__ vec3 operator()(vec3, vec3 p, vec3, vec3 v, float) const {
-        float r = length(p);
+        float r = sqrt_rn_no_tiny(dot(p, p));
         return v - div3_rn_lens(0.3f * p, max(r * r * r, 0.001f));  // 0.3 p / max(r^3, 0.001)
     }
 };
