@@ -177,9 +177,10 @@ __device__ __forceinline__ float quot_rn_signed_zero(float a, float b, float y) 
 // and the FMA forms it exactly even where it is subnormal; the corrected quotient is then the
 // correctly rounded one, as in the normal range.  For s in (2^22, 2^60] the bound stays 2^-60
 // (div3_rn_bounded's); anything else takes the ordinary division.  The test orders magnitudes
-// by their bit patterns doubled minus 2, which sends both zeros to the top.  The captured rays'
-// smaller components take quot_lens_floor where s is the floor 0.001 itself, and the ordinary
-// division elsewhere.  Same result as v / s for every finite p.
+// by their bit patterns doubled minus 2, which sends both zeros to the top.  Where s is the
+// floor 0.001 itself, every component takes quot_lens_floor instead (the captured rays' tiny
+// ones included); a component below 2^-102 above the floor takes the ordinary division.  Same
+// result as v / s for every finite p.
 //
 // quot_lens_floor(a) = a / c correctly rounded, c = RN(0.001) = 0x3a83126f, for |a| <= 2^12 (the
 // lens's components while s == c: r^3 <= c gives |0.3 p_i| <= 0.0301).  With a = j 2^-149:
@@ -201,6 +202,9 @@ __device__ __forceinline__ vec3 div3_rn_lens(vec3 v, float s) {
 #ifdef LINON_FAST_MATH
     return v / s;
 #else
+    // at the floor (r^3 <= 0.001: the lens's centre, where most of its rays are captured) the
+    // constant-divisor form takes every component, tiny or not; most waves are entirely there
+    if (s == 0.001f) return {quot_lens_floor(v.x), quot_lens_floor(v.y), quot_lens_floor(v.z)};
     const unsigned ux = (__float_as_uint(v.x) << 1) - 2u;
     const unsigned uy = (__float_as_uint(v.y) << 1) - 2u;
     const unsigned uz = (__float_as_uint(v.z) << 1) - 2u;
@@ -211,7 +215,6 @@ __device__ __forceinline__ vec3 div3_rn_lens(vec3 v, float s) {
         const float y = recip_rn(s);
         return {quot_rn_signed_zero(v.x, s, y), quot_rn_signed_zero(v.y, s, y), quot_rn_signed_zero(v.z, s, y)};
     }
-    if (s == 0.001f) return {quot_lens_floor(v.x), quot_lens_floor(v.y), quot_lens_floor(v.z)};
     return {v.x / s, v.y / s, v.z / s};
 #endif
 }
