@@ -183,20 +183,22 @@ __device__ __forceinline__ float quot_rn_signed_zero(float a, float b, float y) 
 // result as v / s for every finite p.
 //
 // quot_lens_floor(a) = a / c correctly rounded, c = RN(0.001) = 0x3a83126f, for |a| <= 2^12 (the
-// lens's components while s == c: r^3 <= c gives |0.3 p_i| <= 0.0301).  With a = j 2^-149:
-//  * |j| <= 8388: a / c = (1000 - 4.7497e-5) j 2^-149 is below 2^-126, so it rounds to the
-//    subnormal grid 2^-149, and 4.7497e-5 |j| <= 0.3985 < 1/2 makes the nearest point 1000 j;
-//    1000 a (below 2^23 units of 2^-149, and +-0 for a = +-0) is that point, exactly.
-//  * otherwise a / c >= 2^-126 is normal: a 2^48 (exact) is at least 2^-88 and at most 2^60, in
-//    quot_rn_signed_zero's range with s = c, so RN(a 2^48 / c) is correctly rounded, and its
-//    product with 2^-48 stays normal and exact: RN(a / c).
+// lens's components while s == c: r^3 <= c gives |0.3 p_i| <= 0.0301): Q = RN(a 2^48 / c) by
+// Markstein's correction (a 2^48 is exact and, for a != 0, at least 2^-101 and at most 2^60,
+// in quot_rn_signed_zero's range with s = c), then Q 2^-48, one rounding.
+//  * |a / c| >= 2^-126: Q 2^-48 is normal, so the product is exact and equals RN(a / c).
+//  * otherwise a = j 2^-149 with |j| <= 8388 (a is subnormal: a / c > 999 a), and a / c =
+//    X 2^-149 with X = (1000 - 4.7497e-5) j, 1000 |j| - 0.3985 <= |X| <= 1000 |j| < 2^23.  The
+//    nearest point of the subnormal grid is 1000 j.  Q 2^-48 carries X to 24 bits: within
+//    0.125 + 0.199 of 1000 |j| for |X| < 2^22 (|j| < 4195), and for |X| in [2^22, 2^23) (ulp
+//    1/2, offset in [0.199, 0.3985]) exactly 1000 |j| or 1000 |j| - 1/2.  Rounding it to the
+//    grid therefore gives 1000 j: a distance below 1/2, or a tie that goes to the even 1000 j.
+//  * a = +-0 gives +-0.
 // Y_C = RN(1 / c) = 0x4479ffff, the reciprocal recip_rn(c) forms.
 __device__ __forceinline__ float quot_lens_floor(float a) {
 #pragma clang fp contract(off)
     constexpr float C = 0.001f, Y_C = 999.99993896484375f;
-    const float small = a * 1000.0f;
-    const float big = quot_rn_signed_zero(a * 0x1p48f, C, Y_C) * 0x1p-48f;
-    return __builtin_fabsf(a) <= 8388.0f * 0x1p-149f ? small : big;
+    return quot_rn_signed_zero(a * 0x1p48f, C, Y_C) * 0x1p-48f;
 }
 __device__ __forceinline__ vec3 div3_rn_lens(vec3 v, float s) {
 #ifdef LINON_FAST_MATH
