@@ -1,0 +1,88 @@
+"""CPU checks of the arithmetic arguments the IEEE build's shortcuts rest on (device code in
+linon_amd/csrc/device/wgsl_lib.hpp), in exact rational arithmetic or with float64 standing in
+for an FMA where that is exact.  The GPU self-tests check the device code itself
+(tests/test_gpu_parity.py); these pin the mathematics independently of the device."""
+from fractions import Fraction as F
+
+import numpy as np
+
+C = np.float32(0.001)
+
+
+def _exact(x: np.float32) -> F:
+    b = int(np.float32(x).view(np.uint32))
+    e = (b >> 23) & 0xFF
+    m = b & 0x7FFFFF
+    v = F(m | 0x800000) * F(2) ** (e - 150) if e else F(m) * F(2) ** -149
+    return -v if b >> 31 else v
+
+
+def _round_bits(x: F, bits: int) -> F:
+    """x > 0 rounded to `bits` significant bits, ties to even."""
+    e = 0
+    while x >= 2 ** bits:
+        x /= 2
+        e += 1
+    while x < 2 ** (bits - 1):
+        x *= 2
+        e -= 1
+    n, r = int(x), x - int(x)
+    if r > F(1, 2) or (r == F(1, 2) and n % 2):
+        n += 1
+    return F(n) * F(2) ** e
+
+
+def _round_int(x: F) -> int:
+    n, r = int(x), x - int(x)
+    if r > F(1, 2) or (r == F(1, 2) and n % 2):
+        n += 1
+    return n
+
+
+def test_lens_floor_reciprocal_constant():
+    """Y_C of quot_lens_floor is RN(1 / RN(0.001)): 0x4479ffff."""
+    c = _exact(C)
+    y = np.float32(999.99993896484375)
+    assert int(y.view(np.uint32)) == 0x4479FFFF
+    below = _exact(np.nextafter(y, np.float32(0)))
+    above = _exact(np.nextafter(y, np.float32(2000)))
+    target = 1 / c
+    assert abs(_exact(y) - target) < abs(below - target) and abs(_exact(y) - target) < abs(above - target)
+
+
+def test_lens_floor_subnormal_quotients():
+    """For every subnormal numerator a = j 2^-149 whose quotient a / c is subnormal (j <= 8388),
+    rounding the 24-bit quotient Q = RN(a 2^48 / c) once more to the subnormal grid gives the
+    correctly rounded a / c, and that is 1000 j; j = 8389 is the first normal quotient."""
+    c = _exact(C)
+    for j in range(1, 8389):
+        x = F(j) / c  # a / c in units of 2^-149
+        assert x < 2 ** 23
+        assert _round_int(x) == 1000 * j
+        assert _round_int(_round_bits(x, 24)) == 1000 * j, j
+    assert F(8389) / c >= 2 ** 23
+
+
+def test_refraction_k_gap():
+    """k = RN(1 - e t) (one FMA) for floats e and t in [0, 1] is never in (0, 2^-48): near
+    e t = 1 the product is a multiple of 2^-48.  float64 forms e t (48-bit product) and 1 - e t
+    (within a factor 2 of 1) exactly here, so float32 of it is the FMA.  Drawn near the critical
+    product: e = 1 + a 2^-23, t = 1 - b 2^-24 with b within 3 of 2a (e t = 1 + (2a - b) 2^-24 -
+    a b 2^-47), and random e in [0.5, 2) with t within 16 ulps of 1 / e."""
+    a = np.arange(0, 4096, dtype=np.int64)
+    d = np.arange(-3, 4, dtype=np.int64)
+    A = np.repeat(a, d.size)
+    B = np.clip(2 * A + np.tile(d, a.size), 0, 2 ** 23)
+    e1 = (np.float64(1.0) + A * 2.0 ** -23).astype(np.float32)
+    t1 = (np.float64(1.0) - B * 2.0 ** -24).astype(np.float32)
+    rng = np.random.default_rng(7)
+    e2 = (0.5 + 1.5 * rng.random(400_000)).astype(np.float32)
+    t0 = (np.float32(1.0) / e2).astype(np.float32)
+    off = rng.integers(-16, 17, size=e2.size).astype(np.int64)
+    t2 = (t0.view(np.uint32).astype(np.int64) + off).astype(np.uint32).view(np.float32)
+    e = np.concatenate([e1, e2])
+    t = np.minimum(np.concatenate([t1, t2]), np.float32(1.0))
+    k = (1.0 - e.astype(np.float64) * t.astype(np.float64)).astype(np.float32)
+    assert not ((k != 0) & (np.abs(k) < np.float32(2.0 ** -48))).any()
+    nz = np.abs(k[k != 0])
+    assert (k == 0).any() and nz.min() <= 2.0 ** -46  # the critical cases occur in the draw
