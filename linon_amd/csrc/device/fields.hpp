@@ -93,7 +93,15 @@ struct FieldLens {  // v - 0.3 * p / max(|p|^3, 1e-3)
     // s the floor 0.001 either way; elsewhere sqrt_rn_no_tiny is sqrt_rn.  So |p| needs no
     // rescaling of tiny operands, and s (hence the field) is bit for bit length()'s.
     __device__ __forceinline__ vec3 operator()(vec3, vec3 p, vec3, vec3 v, float) const {
-        float r = sqrt_rn_no_tiny(dot(p, p));
+        const float d2 = dot(p, p);
+#ifndef LINON_FAST_MATH
+        // The hardware root is within an ulp of |p|: below 0.0999 on every lane of the wave
+        // (|p|^3 < 0.000998 however |p| rounds), s is the floor throughout and the wave skips
+        // the correctly rounded root and the cube (a wave-uniform branch: the lens's captured
+        // centre, where most waves are).  NaN fails the test and takes the general path.
+        if (__all(__builtin_amdgcn_sqrtf(d2) < 0.0999f)) return v - div3_rn_lens_floor(0.3f * p);
+#endif
+        const float r = sqrt_rn_no_tiny(d2);
         return v - div3_rn_lens(0.3f * p, max(r * r * r, 0.001f));  // 0.3 p / max(r^3, 0.001)
     }
 };

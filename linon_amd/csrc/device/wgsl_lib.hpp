@@ -200,13 +200,17 @@ __device__ __forceinline__ float quot_lens_floor(float a) {
     constexpr float C = 0.001f, Y_C = 999.99993896484375f;
     return quot_rn_signed_zero(a * 0x1p48f, C, Y_C) * 0x1p-48f;
 }
+// div3_rn_lens(v, 0.001) for |v_i| <= 2^12
+__device__ __forceinline__ vec3 div3_rn_lens_floor(vec3 v) {
+    return {quot_lens_floor(v.x), quot_lens_floor(v.y), quot_lens_floor(v.z)};
+}
 __device__ __forceinline__ vec3 div3_rn_lens(vec3 v, float s) {
 #ifdef LINON_FAST_MATH
     return v / s;
 #else
     // at the floor (r^3 <= 0.001: the lens's centre, where most of its rays are captured) the
     // constant-divisor form takes every component, tiny or not; most waves are entirely there
-    if (s == 0.001f) return {quot_lens_floor(v.x), quot_lens_floor(v.y), quot_lens_floor(v.z)};
+    if (s == 0.001f) return div3_rn_lens_floor(v);
     const unsigned ux = (__float_as_uint(v.x) << 1) - 2u;
     const unsigned uy = (__float_as_uint(v.y) << 1) - 2u;
     const unsigned uz = (__float_as_uint(v.z) << 1) - 2u;
