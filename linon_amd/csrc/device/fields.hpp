@@ -120,7 +120,9 @@ struct FieldHotAirT {
         return FaithfulRoot ? __builtin_amdgcn_sqrtf(dot(d, d)) : length(d);
     }
     __device__ __forceinline__ static float temperature(vec3 q) {
-        const float x = 8.0f * (0.5f - distance(q));
+        // 8 (0.5 - d) = -RN(8 d - 4) exactly: scaling by 8 commutes with the rounding, so one
+        // FMA forms the product of the rounded difference
+        const float x = -__builtin_fmaf(8.0f, distance(q), -4.0f);
 #ifdef LINON_FAST_MATH
         return 20.0f + 80.0f * sigmoid_floor(x);  // in [20, 100]
 #else
