@@ -371,6 +371,10 @@ __device__ __forceinline__ float refraction_index(float t) { return refraction_i
 // InRange: temperatures in [-250, 1e4] deg C (or NaN).  There the index's numerator lies in
 // [0.95, 1.06] and its denominator in [0.085, 37.6], so both indices lie in [0.025, 12.5] and
 // every quotient has its operands in div_in_range's range.
+__device__ __forceinline__ float mul_rounded(float a, float b) {  // a product the compiler may not fuse
+#pragma clang fp contract(off)
+    return a * b;
+}
 template <bool InRange>
 __device__ __forceinline__ vec3 refraction_impl(float t_in, float t_out, vec3 v_in, vec3 n) {
     auto quot = [](float a, float b) { return InRange ? div_in_range(a, b) : a / b; };
@@ -390,8 +394,21 @@ __device__ __forceinline__ vec3 refraction_impl(float t_in, float t_out, vec3 v_
     // exponents of e and t add up to at least -2 and e t is a multiple of 2^-48, as is 1 - e t.
     // The root therefore needs no rescaling of tiny operands.
     const float k = __builtin_fmaf(-(eta * eta), __builtin_fmaf(-cosi, cosi, 1.0f), 1.0f);
-    if (k < 0.0f) return reflect(v_in, n_ref);  // total internal reflection
-    return eta * v_in + (eta * cosi - sqrt_rn_no_tiny(k)) * n_ref;
+    // eta v_in + (eta cosi - sqrt(k)) n_ref with its contraction written out, fma(eta, v_i,
+    // RN(X n_i)), X = fma(eta, cosi, -sqrt(k)): the form the compiler picks for the expression, so
+    // that it no longer depends on that pick.  NaN where k < 0.
+    const float X = __builtin_fmaf(eta, cosi, -sqrt_rn_no_tiny(k));
+    vec3 out = {__builtin_fmaf(eta, v_in.x, mul_rounded(X, n_ref.x)),
+                __builtin_fmaf(eta, v_in.y, mul_rounded(X, n_ref.y)),
+                __builtin_fmaf(eta, v_in.z, mul_rounded(X, n_ref.z))};
+    // total internal reflection (k < 0) on any lane: a wave-uniform branch, so that waves
+    // without it (nearly all) carry no reflection code, not even with an empty lane mask
+    if (__any(k < 0.0f)) {
+        const vec3 r = reflect(v_in, n_ref);
+        const bool tir = k < 0.0f;
+        out = {tir ? r.x : out.x, tir ? r.y : out.y, tir ? r.z : out.z};
+    }
+    return out;
 }
 __device__ __forceinline__ vec3 refraction(float t_in, float t_out, vec3 v_in, vec3 n) {
     return refraction_impl<false>(t_in, t_out, v_in, n);
@@ -423,9 +440,9 @@ __device__ __forceinline__ vec3 refraction_in_range_same(float t, vec3 v_in, vec
         n_ref = -n;
     }
     // 1 - t with t = RN(1 - cosi cosi) in [0, 1]: zero, or at least 2^-24 (t >= 1/2 is a
-    // multiple of 2^-24) -- no tiny operand for the root
+    // multiple of 2^-24) -- no tiny operand for the root; never negative, so never the total
+    // internal reflection of the general form (a NaN t gives NaN there as here)
     const float k = 1.0f - __builtin_fmaf(-cosi, cosi, 1.0f);
-    if (k < 0.0f) return reflect(v_in, n_ref);
     const float x = cosi - sqrt_rn_no_tiny(k);
     auto axpy = [](float a, float b, float c) {
 #pragma clang fp contract(off)

@@ -323,6 +323,25 @@ __global__ void __launch_bounds__(256) div_selftest_refraction_kernel(unsigned l
     bad += first_stage_mismatches(lw::FieldMiragePlane{}, nrm, v);
     bad += first_stage_mismatches(lw::FieldMirageSphericalSigmoid{}, nrm, v);
     bad += first_stage_mismatches(lw::FieldMiragePlaneSigmoid{}, nrm, v);
+    // refraction_in_range against the reference's branches restated here (src/main_view.wgsl:
+    // 119-135): the side of the normal, and reflection where k < 0 (frequent on these draws)
+    {
+        using namespace lw;
+        const float ei = div_in_range(refraction_index_num(t_in), refraction_index_den(t_in));
+        const float eo = div_in_range(refraction_index_num(t_out), refraction_index_den(t_out));
+        float cosi = clamp(-1.0f, 1.0f, dot(v, nrm));
+        vec3 n_ref = nrm;
+        if (cosi < 0.0f) cosi = -cosi;
+        else n_ref = -nrm;
+        const float eta = div_in_range(ei, eo);
+        const float k = __builtin_fmaf(-(eta * eta), __builtin_fmaf(-cosi, cosi, 1.0f), 1.0f);
+        const float X = __builtin_fmaf(eta, cosi, -sqrt_rn(k));
+        const vec3 refr = {__builtin_fmaf(eta, v.x, mul_rounded(X, n_ref.x)),
+                           __builtin_fmaf(eta, v.y, mul_rounded(X, n_ref.y)),
+                           __builtin_fmaf(eta, v.z, mul_rounded(X, n_ref.z))};
+        const vec3 want = k < 0.0f ? reflect(v, n_ref) : refr;
+        bad += !same_or_nan(b.x, want.x) + !same_or_nan(b.y, want.y) + !same_or_nan(b.z, want.z);
+    }
     // refraction's k = RN(1 - e t) where e t is near 1 (e in [0.5, 2), t within 8 ulps of 1 / e,
     // at most 1): never in (0, 2^-48), and its root without rescaling equals sqrt_rn's
     const float e2 = 0.5f + 1.5f * rand_unit(r[2] ^ r[5]);
@@ -467,7 +486,7 @@ hipError_t run_division_selftest(unsigned long long random_pairs, unsigned long 
             const unsigned long long m = random_pairs - off < chunk ? random_pairs - off : chunk;
             hipLaunchKernelGGL(div_selftest_refraction_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
                                nullptr, 0x5851F42D4C957F2Dull + off, m, d + 1);
-            n += 26 * m;
+            n += 29 * m;
         }
         for (unsigned long long off = 0; off < random_pairs && e == hipSuccess; off += chunk) {
             const unsigned long long m = random_pairs - off < chunk ? random_pairs - off : chunk;
