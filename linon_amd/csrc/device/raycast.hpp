@@ -476,12 +476,16 @@ __device__ __forceinline__ bool ray_finished(const RayState& r, const StepParams
 // WOne: the field weight is 1 (every benchmark configuration).  The blend (1 - w) d + w V then
 // is RN(RN(0 d) + V), which one FMA fma(0, d, V) rounds identically (0 d is exact), so the
 // step loop saves two of its three operations per component.
-template <class F, bool WOne>
+// Fixed: 0 reads the adaptive flag and the mode from I; 1 and 2 are the fixed-step loops of
+// the reference mode and of the escape mode, compiled without the other branches, so the step
+// loop holds fewer uniform flags (fewer scalar registers to spill and reload per step).
+enum : int { LK_STEP_ANY = 0, LK_STEP_FIXED_REFERENCE = 1, LK_STEP_FIXED_ESCAPE = 2 };
+template <class F, bool WOne, int Fixed>
 __device__ __forceinline__ bool ray_advance_impl(RayState& r, const StepParams& I, const SceneParams& S,
                                                  const float4* lds, int budget, QueryStats& st) {
     const F field{};
-    const bool adaptive = I.adaptive != 0;
-    const bool escape = (I.mode == LINON_MODE_ESCAPE);
+    const bool adaptive = Fixed == LK_STEP_ANY ? I.adaptive != 0 : false;
+    const bool escape = Fixed == LK_STEP_ANY ? (I.mode == LINON_MODE_ESCAPE) : Fixed == LK_STEP_FIXED_ESCAPE;
     const bool brute = (S.accel == LINON_ACCEL_BRUTE_K);
     const float margin = S.skip_margin;
     const float w = I.field_weight;
@@ -593,8 +597,14 @@ __device__ __forceinline__ bool ray_advance_impl(RayState& r, const StepParams& 
 template <class F>
 __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, const SceneParams& S,
                                             const float4* lds, int budget, QueryStats& st) {
-    if (I.field_weight == 1.0f) return ray_advance_impl<F, true>(r, I, S, lds, budget, st);
-    return ray_advance_impl<F, false>(r, I, S, lds, budget, st);
+    if (I.field_weight == 1.0f) {
+        if (I.adaptive == 0 && I.mode == LINON_MODE_REF)
+            return ray_advance_impl<F, true, LK_STEP_FIXED_REFERENCE>(r, I, S, lds, budget, st);
+        if (I.adaptive == 0 && I.mode == LINON_MODE_ESCAPE)
+            return ray_advance_impl<F, true, LK_STEP_FIXED_ESCAPE>(r, I, S, lds, budget, st);
+        return ray_advance_impl<F, true, LK_STEP_ANY>(r, I, S, lds, budget, st);
+    }
+    return ray_advance_impl<F, false, LK_STEP_ANY>(r, I, S, lds, budget, st);
 }
 
 // Colour (alpha forced to 1, black when nothing was hit) and end point of a finished ray

@@ -12,22 +12,8 @@
 
 namespace LINON_KERNEL_NS {
 
-// Waves per SIMD per field: swirl, whose short step leaves the latency of its transcendentals
-// exposed, is built for 7 (72 VGPRs; about 2% faster than at 6 on its 1080p frame); the other
-// fields for LK_RAYCAST_WAVES_PER_EU (6: lens and hot air are slower at 5 and at 7, warp the
-// same at 7).
 template <class F>
-struct raycast_waves {
-    static constexpr int value = LK_RAYCAST_WAVES_PER_EU;
-};
-template <>
-struct raycast_waves<lw::FieldSwirl> {
-    static constexpr int value = 7;
-};
-
-template <class F>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(raycast_waves<F>::value)))
-raycast_kernel(RaycastParams P) {
+__global__ void __launch_bounds__(64) LK_RAYCAST_OCCUPANCY raycast_kernel(RaycastParams P) {
     lk::raycast_body<F>(P);
 }
 
