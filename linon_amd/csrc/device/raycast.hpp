@@ -479,10 +479,16 @@ __device__ __forceinline__ bool ray_finished(const RayState& r, const StepParams
 // Fixed: 0 reads the adaptive flag and the mode from I; 1 and 2 are the fixed-step loops of
 // the reference mode and of the escape mode, compiled without the other branches, so the step
 // loop holds fewer uniform flags (fewer scalar registers to spill and reload per step).
+// Stop: a predicate on the ray after each step; when it holds the loop returns early, with
+// ray_finished's verdict (the approach leg of a surrogate frame stops at the sphere).
 enum : int { LK_STEP_ANY = 0, LK_STEP_FIXED_REFERENCE = 1, LK_STEP_FIXED_ESCAPE = 2 };
-template <class F, bool WOne, int Fixed>
+struct NeverStop {
+    __device__ __forceinline__ bool operator()(const RayState&) const { return false; }
+};
+template <class F, bool WOne, int Fixed, class Stop = NeverStop>
 __device__ __forceinline__ bool ray_advance_impl(RayState& r, const StepParams& I, const SceneParams& S,
-                                                 const float4* lds, int budget, QueryStats& st) {
+                                                 const float4* lds, int budget, QueryStats& st,
+                                                 Stop stop = Stop{}) {
     const F field{};
     const bool adaptive = Fixed == LK_STEP_ANY ? I.adaptive != 0 : false;
     const bool escape = Fixed == LK_STEP_ANY ? (I.mode == LINON_MODE_ESCAPE) : Fixed == LK_STEP_FIXED_ESCAPE;
@@ -590,21 +596,22 @@ __device__ __forceinline__ bool ray_advance_impl(RayState& r, const StepParams& 
                 return true;
             }
         }
+        if (stop(r)) return ray_finished(r, I);
     }
     return ray_finished(r, I);
 }
 
-template <class F>
+template <class F, class Stop = NeverStop>
 __device__ __forceinline__ bool ray_advance(RayState& r, const StepParams& I, const SceneParams& S,
-                                            const float4* lds, int budget, QueryStats& st) {
+                                            const float4* lds, int budget, QueryStats& st, Stop stop = Stop{}) {
     if (I.field_weight == 1.0f) {
         if (I.adaptive == 0 && I.mode == LINON_MODE_REF)
-            return ray_advance_impl<F, true, LK_STEP_FIXED_REFERENCE>(r, I, S, lds, budget, st);
+            return ray_advance_impl<F, true, LK_STEP_FIXED_REFERENCE>(r, I, S, lds, budget, st, stop);
         if (I.adaptive == 0 && I.mode == LINON_MODE_ESCAPE)
-            return ray_advance_impl<F, true, LK_STEP_FIXED_ESCAPE>(r, I, S, lds, budget, st);
-        return ray_advance_impl<F, true, LK_STEP_ANY>(r, I, S, lds, budget, st);
+            return ray_advance_impl<F, true, LK_STEP_FIXED_ESCAPE>(r, I, S, lds, budget, st, stop);
+        return ray_advance_impl<F, true, LK_STEP_ANY>(r, I, S, lds, budget, st, stop);
     }
-    return ray_advance_impl<F, false, LK_STEP_ANY>(r, I, S, lds, budget, st);
+    return ray_advance_impl<F, false, LK_STEP_ANY>(r, I, S, lds, budget, st, stop);
 }
 
 // Colour (alpha forced to 1, black when nothing was hit) and end point of a finished ray

@@ -70,6 +70,11 @@ struct ApproachParams {
     int q_is_v0;            // 1: the field reads v0 (q = v0), 0: it reads v (q = current direction)
 };
 
+struct InsideSphere {
+    float r2;
+    __device__ __forceinline__ bool operator()(const lk::RayState& r) const { return lk::rn::dot(r.p, r.p) < r2; }
+};
+
 // Integrates a camera ray until it is inside the sphere (|p| < R) or finished (escape test,
 // max_steps).  Fields act outside the sphere too, so this leg is plain RK4.
 template <class F>
@@ -93,8 +98,9 @@ __device__ void approach_body(const ApproachParams& A) {
     bool entered = lk::rn::dot(r.p, r.p) < r2;
     bool done = false;
     lk::QueryStats st{0ull, 0u, 0u, 0u};  // no mesh: nothing is counted
-    while (!entered && !done) {
-        done = lk::ray_advance<F>(r, A.step, none, nullptr, 1, st);
+    if (!entered) {
+        // steps until the ray is inside the sphere (the loop's own stop test) or finished
+        done = lk::ray_advance<F>(r, A.step, none, nullptr, 0x7fffffff, st, InsideSphere{r2});
         entered = lk::rn::dot(r.p, r.p) < r2;
     }
     const size_t texel = (size_t)(A.cam.height - 1 - gy) * A.cam.width + gx;
