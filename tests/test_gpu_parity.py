@@ -56,6 +56,27 @@ def test_config_camera_cornell(view, field):
     _case(view, 64, 64, wl.camera(), wl.settings(), wl.integrator(), field, v, f, f"cornell-{field}")
 
 
+@pytest.mark.parametrize("weight", [0.5, 0.0])
+@pytest.mark.parametrize("field", ["swirl", "hot_air"])
+def test_field_weight_blend(view, field, weight):
+    """The blend cur_dir = (1 - w) cur_dir + w V (src/main_view.wgsl:255) for a field weight
+    other than 1, which runs the step loop's general form (the measured configurations all have
+    w = 1); w = 0 ignores the field and draws straight rays."""
+    wl = configs.PLAIN_256_SWIRL
+    v, f = wl.mesh()
+    st = L.Settings.make(field_weight=weight)
+    gpu, ref = _case(view, 64, 64, wl.camera(), st, wl.integrator(), field, v, f, f"weight{weight}-{field}")
+    if weight == 0.0:  # straight rays: the identity field's frame, bit for bit, in one build
+        view.set_math(L.MATH_IEEE)  # (the default runs swirl and the identity in the WGSL build)
+        try:
+            a = run_gpu(view, wl.camera(), st, wl.integrator(), field, v, f)
+            b = run_gpu(view, wl.camera(), st, wl.integrator(), "identity", v, f)
+        finally:
+            view.set_math(L.MATH_AUTO)
+        assert np.array_equal(a.hit_faces, b.hit_faces)
+        assert np.array_equal(a.mapping, b.mapping)
+
+
 @pytest.mark.parametrize("math", ["ieee", "wgsl"])
 @pytest.mark.parametrize("field", ["swirl", "lens", "warp"])
 def test_math_builds(view, field, math):
