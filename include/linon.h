@@ -375,7 +375,10 @@ linon_status linon_flowmap_noalloc(int device, int field, int math, const linon_
  * normalize and the lens field (lw::div3_rn_bounded, lw::div3_rn_lens) at every numerator
  * mantissa of 18 exponents from the subnormals up against 19 denominators, and the lens
  * quotient at its floor s = 0.001 (lw::quot_lens_floor) at every numerator below 2^13 in
- * magnitude.  quotient_mismatches counts all but the reciprocal's.  Writes the mismatch counts and the number of
+ * magnitude; the RK4 average's division by six at every float, apart from its ties of subnormal
+ * quotients s = (6 m + 3) 2^-149 with m odd, where it returns the other neighbour, and s = +-inf,
+ * where it returns NaN.
+ * quotient_mismatches counts all but the reciprocal's.  Writes the mismatch counts and the number of
  * comparisons. */
 linon_status linon_selftest_division(int device, uint64_t random_pairs, uint64_t* recip_mismatches,
                                      uint64_t* quotient_mismatches, uint64_t* comparisons);

@@ -251,13 +251,14 @@ __device__ __forceinline__ void query_grid(const SceneParams& P, vec3 o, vec3 d,
 }
 
 // (k1 + 2 k2 + 2 k3 + k4) / 6.0 with the correctly rounded quotient of the WGSL division:
-// q = s * RN(1/6), then one FMA residual correction (exact for every |s| above 2^-102), so
-// that e.g. a uniform field keeps V == v bit for bit as the f32 division does.
+// q = s * RN(1/6), then one FMA residual correction in the signed-zero form
+// (lw::quot_rn_signed_zero, which also keeps -0 / 6 = -0), so that e.g. a uniform field keeps
+// V == v bit for bit as the f32 division does.  Equal to s / 6 for every s except subnormal
+// quotients at an exact tie (s = 3 (2m + 1) 2^-149 with m odd), where RN(1/6) > 1/6 tips the
+// correction to the other neighbour (an ulp of 2^-149 in V), and s = +-inf, where the residual
+// is inf - inf and V NaN (a field value the ray does not survive either way).
 __device__ __forceinline__ float div6(float s) {
-    const float inv6 = 1.0f / 6.0f;
-    float q = s * inv6;
-    float r = __builtin_fmaf(-q, 6.0f, s);
-    return __builtin_fmaf(r, inv6, q);
+    return lw::quot_rn_signed_zero(s, 6.0f, 1.0f / 6.0f);
 }
 __device__ __forceinline__ vec3 rk4_average(vec3 k1, vec3 k2, vec3 k3, vec3 k4) {
     vec3 s = ((k1 + 2.0f * k2) + 2.0f * k3) + k4;

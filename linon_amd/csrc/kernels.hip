@@ -274,6 +274,25 @@ __global__ void __launch_bounds__(256) exp_selftest_kernel(unsigned hi, unsigned
     atomicMax(max_milli_ulp, (unsigned)fmin(err * 1024.0, 4.0e9));
 }
 
+// every float s whose bit pattern has the top byte `hi`: the RK4 average's division by six
+// (lk::div6) against s / 6, except its documented exceptions: the ties of subnormal quotients
+// (s = (6 m + 3) 2^-149 with m odd, below 1.5 2^-124), where div6 returns the other
+// neighbour, and s = +-inf, where its residual is inf - inf and the result NaN
+__global__ void __launch_bounds__(256) div6_selftest_kernel(unsigned hi, unsigned long long* mism) {
+    const unsigned lo = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lo >= (1u << 24)) return;
+    const float s = __uint_as_float((hi << 24) | lo);
+    const float got = lk::div6(s), want = s / 6.0f;
+    if (same_or_nan(got, want)) return;
+    const float a = __builtin_fabsf(s);
+    bool tie = a == __builtin_inff();
+    if (a < 0x1.8p-124f) {
+        const unsigned long long j = (unsigned long long)((double)a * 0x1p149);  // exact
+        tie = j % 6 == 3 && ((j - 3) / 6) % 2 == 1;
+    }
+    if (!tie) atomicAdd(mism, 1ull);
+}
+
 // every float x whose bit pattern has the top byte `hi`: sqrt_rn(x), and sqrt_rn_no_tiny(x)
 // where |x| is not in (0, 2^-96), against the compiler's correctly rounded square root
 __global__ void __launch_bounds__(256) sqrt_selftest_kernel(unsigned hi, unsigned long long* mism) {
@@ -472,6 +491,10 @@ hipError_t run_division_selftest(unsigned long long random_pairs, unsigned long 
             hipLaunchKernelGGL(div_selftest_bounded_kernel, dim3(mblocks), dim3(256), 0, nullptr, ea, b, d + 1);
             n += 12ull << 23;
         }
+    for (unsigned hi = 0; hi < 256 && e == hipSuccess; ++hi) {  // div6 at every float
+        hipLaunchKernelGGL(div6_selftest_kernel, dim3((1u << 24) / 256), dim3(256), 0, nullptr, hi, d + 1);
+        n += 1ull << 24;
+    }
     for (int be = 0; be < 127 + 13 && e == hipSuccess; ++be) {  // lens floor: |a| < 2^13
         hipLaunchKernelGGL(div_selftest_lens_floor_kernel, dim3(mblocks), dim3(256), 0, nullptr, be, d + 1);
         n += 4ull << 23;

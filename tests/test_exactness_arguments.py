@@ -86,3 +86,36 @@ def test_refraction_k_gap():
     assert not ((k != 0) & (np.abs(k) < np.float32(2.0 ** -48))).any()
     nz = np.abs(k[k != 0])
     assert (k == 0).any() and nz.min() <= 2.0 ** -46  # the critical cases occur in the draw
+
+
+def _div6_device_form(s: np.ndarray) -> np.ndarray:
+    """div6 of the ray-cast kernels (q0 = RN(s y), r' = RN(6 q0 - s), q = RN(q0 - r' y), y =
+    RN(1/6)) for |s| < 1.5 2^-124, each FMA formed in float64 and rounded once to float32.  At
+    these magnitudes every operand is a multiple of 2^-149 (or y's 2^-26) with few significant
+    bits, so float64 forms the FMAs exactly."""
+    y = np.float32(1.0) / np.float32(6.0)
+    q0 = (s.astype(np.float64) * np.float64(y)).astype(np.float32)
+    rn = (6.0 * q0.astype(np.float64) - s.astype(np.float64)).astype(np.float32)
+    return (q0.astype(np.float64) - rn.astype(np.float64) * np.float64(y)).astype(np.float32)
+
+
+def test_rk4_division_by_six():
+    """div6 equals IEEE s / 6 (zeros' signs included) at every s whose quotient is subnormal
+    (|s| < 1.5 2^-124, both signs) except the documented ties: s = (6 m + 3) 2^-149 with m odd,
+    where it returns m 2^-149 instead of the even (m + 1) 2^-149.  (The device self-test compares
+    the device's own div6 with the division at every float.)"""
+    top = int(np.float32(1.5 * 2.0 ** -124).view(np.uint32))
+    ties = 0
+    for lo in range(0, top, 1 << 23):
+        bits = np.arange(lo, min(lo + (1 << 23), top), dtype=np.uint32)
+        for sign in (0, 0x80000000):
+            sv = (bits | np.uint32(sign)).view(np.float32)
+            got = _div6_device_form(sv)
+            want = sv / np.float32(6.0)
+            bad = got.view(np.uint32) != want.view(np.uint32)
+            jj = np.rint(np.abs(sv.astype(np.float64)) * 2.0 ** 149).astype(np.int64)  # s = j 2^-149
+            tie_odd = (jj % 6 == 3) & (((jj - 3) // 6) % 2 == 1)
+            assert np.array_equal(bad, tie_odd), lo
+            assert (np.abs(want[bad].astype(np.float64) - got[bad].astype(np.float64)) == 2.0 ** -149).all()
+            ties += int(bad.sum())
+    assert ties > 0

@@ -373,8 +373,10 @@ def test_shared_denominator_division_is_ieee():
     must equal the field evaluated at p_prev = p.  The bounded divisions of normalize and of
     the lens field (whose relaxed range admits zeros and numerators down to 2^-102) are also
     checked at every numerator mantissa of 18 exponents from the subnormals up to 2^0, beside
-    signed zeros, against 19 denominators in and out of their ranges; and the lens quotient at
-    its floor s = 0.001 (the captured rays' path) at every numerator of magnitude below 2^13."""
+    signed zeros, against 19 denominators in and out of their ranges; the lens quotient at
+    its floor s = 0.001 (the captured rays' path) at every numerator of magnitude below 2^13; and
+    the RK4 average's division by six at every float (apart from its documented ties of
+    subnormal quotients)."""
     print("[case] division self-test", flush=True)
     res = L.selftest_division(random_pairs=1 << 28)
     print(res, flush=True)
