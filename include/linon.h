@@ -388,11 +388,11 @@ linon_status linon_selftest_division(int device, uint64_t random_pairs, uint64_t
  * operands (lw::sqrt_rn_no_tiny) at every float whose magnitude is not in (0, 2^-96). */
 linon_status linon_selftest_sqrt(int device, uint64_t* mismatches, uint64_t* comparisons);
 /* Diagnostics (no counterpart in the reference): the exponential of hot air's default variant
- * (lw::exp_faithful) at every float in [-87, 87] against the FP64 exponential, in units of the
- * float spacing at the exact result: how many results are off by more than 2 of them, how many
- * floats were compared (every float bit pattern is visited; those outside the range are
- * skipped), and the largest error. */
-linon_status linon_selftest_exp(int device, uint64_t* over_2ulp, uint64_t* comparisons, float* max_ulp);
+ * (lw::exp_wgsl, the hardware 2^t of RN(y log2 e)) at every float y in [-87, 87] against the
+ * FP64 exponential, in units of the float spacing at the exact result: how many results are off
+ * by more than WGSL's bound for exp, 3 + 2 |y| of them, how many floats were compared (every
+ * float bit pattern is visited; those outside the range are skipped), and the largest error. */
+linon_status linon_selftest_exp(int device, uint64_t* over_bound, uint64_t* comparisons, float* max_ulp);
 /* Diagnostics (no counterpart in the reference): the device's measured rate of non-packed FP32
  * FMA (v_fma_f32, 2 FLOP each), every lane running 16 independent chains over all compute
  * units; the attainable FP32 rate of kernels built without packed FP32. */

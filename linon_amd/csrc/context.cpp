@@ -542,13 +542,13 @@ linon_status linon_selftest_sqrt(int device, uint64_t* mismatches, uint64_t* com
     return LINON_OK;
 }
 
-linon_status linon_selftest_exp(int device, uint64_t* over_2ulp, uint64_t* comparisons, float* max_ulp) {
-    if (!over_2ulp || !comparisons || !max_ulp) return LINON_ERR_ARGUMENT;
+linon_status linon_selftest_exp(int device, uint64_t* over_bound, uint64_t* comparisons, float* max_ulp) {
+    if (!over_bound || !comparisons || !max_ulp) return LINON_ERR_ARGUMENT;
     if (hipSetDevice(device) != hipSuccess) return LINON_ERR_HIP;
     unsigned long long o = 0, n = 0;
     float m = 0.0f;
     if (linon::run_exp_selftest(&o, &n, &m) != hipSuccess) return LINON_ERR_HIP;
-    *over_2ulp = o;
+    *over_bound = o;
     *comparisons = n;
     *max_ulp = m;
     return LINON_OK;

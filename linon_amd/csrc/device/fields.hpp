@@ -106,12 +106,14 @@ struct FieldLens {  // v - 0.3 * p / max(|p|^3, 1e-3)
     }
 };
 // Hot air: refraction(T(p_prev), T(p), v, normalize(p - c)), T = 20 + 80 sigmoid(8 (0.5 - |p - c|)).
-// FaithfulRoot: the distance |p - c| (the temperature's argument and the normal's length) takes
-// the hardware square root, within 1 ulp (WGSL bounds sqrt only to inverseSqrt's accuracy),
-// instead of the correctly rounded one, and the sigmoid's exponential exp_faithful (within
-// about 1.5 ulp) instead of the full-precision library's: the variant AUTO picks for hot air.
-// Its exit points are as close to the FP64 oracle as the exact variant's, and the step is 11%
-// shorter (the correctly rounded root takes 14 instructions, the library exponential 13).
+// FaithfulRoot: the variant AUTO picks for hot air.  Its temperature and normal are taken at
+// WGSL's accuracy: the distance |p - c| (the temperature's argument and the normal's length)
+// with the hardware square root, within 1 ulp (WGSL bounds sqrt only to inverseSqrt's
+// accuracy); the sigmoid with the hardware exponential and reciprocal (sigmoid_wgsl); the
+// normal's quotients through one correctly rounded reciprocal.  Refraction (the index ratio and
+// sqrt(k), where rounding turns into exit error near grazing incidence) stays correctly
+// rounded.  Its exit points pass every parity test as the exact variant's do, in 30% fewer
+// instructions per step.
 template <bool FaithfulRoot>
 struct FieldHotAirT {
     __device__ __forceinline__ static vec3 offset(vec3 q) { return q - vec3f(0.0f, -0.5f, 0.0f); }
@@ -126,11 +128,20 @@ struct FieldHotAirT {
 #ifdef LINON_FAST_MATH
         return 20.0f + 80.0f * sigmoid_floor(x);  // in [20, 100]
 #else
-        return 20.0f + 80.0f * (FaithfulRoot ? sigmoid_floor_faithful(x) : sigmoid_floor(x));
+        return 20.0f + 80.0f * (FaithfulRoot ? sigmoid_wgsl(x) : sigmoid_floor(x));
 #endif
     }
+    // normalize(p - c).  The faithful variant multiplies by the correctly rounded reciprocal of
+    // its faithful |p - c|: RN(d_i RN(1 / s)) is within about one ulp of d_i / s (two roundings
+    // of half an ulp), where WGSL bounds normalize by x / length(x), a 2.5-ulp division.  No
+    // range test: s = 0 gives NaN components as 0 / 0 does.
     __device__ __forceinline__ static vec3 normal(vec3 p) {
-        return FaithfulRoot ? div3_rn_bounded(offset(p), distance(p)) : normalize(offset(p));
+        if (FaithfulRoot) {
+            const vec3 d = offset(p);
+            const float y = recip_rn(distance(p));
+            return {d.x * y, d.y * y, d.z * y};
+        }
+        return normalize(offset(p));
     }
     __device__ __forceinline__ vec3 operator()(vec3 p_prev, vec3 p, vec3, vec3 v, float) const {
         return refraction_in_range(temperature(p_prev), temperature(p), v, normal(p));

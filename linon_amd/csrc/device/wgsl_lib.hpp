@@ -468,22 +468,14 @@ __device__ __forceinline__ float sigmoid_floor(float x) {
     return recip_rn(b > 0x1p125f ? 0x1p125f : b);
 #endif
 }
-// e^y within about 1.5 ulp for y <= 87 (beyond, the callers clamp): 2^(y log2 e) with the
-// product split into RN(y L) + a low part (the FMA residual and y times the rounding error of
-// L = RN(log2 e)), the hardware 2^t of the high part, and a first-order correction
-// 2^lo = 1 + lo ln 2.  Six instructions against the full-precision library's thirteen.
-__device__ __forceinline__ float exp_faithful(float y) {
-    const float L = 1.44269502f, L_lo = 1.925963e-08f, LN2 = 0.693147182f;
-    const float t = y * L;
-    const float lo = __builtin_fmaf(y, L_lo, __builtin_fmaf(y, L, -t));
-    const float r = __builtin_amdgcn_exp2f(t);
-    return __builtin_fmaf(r * lo, LN2, r);
-}
-// sigmoid_floor with exp_faithful: e^-x is taken of min(-x, 87), where e^-x already exceeds
-// the 2^125 the reciprocal's operand is clamped to, so the clamp decides there as before.
-__device__ __forceinline__ float sigmoid_floor_faithful(float x) {
-    const float b = 1.0f + exp_faithful(fminf(-x, 87.0f));
-    return recip_rn(b > 0x1p125f ? 0x1p125f : b);
-}
+// e^y at WGSL's accuracy: the hardware 2^t of t = RN(y L), L = RN(log2 e).  The product's
+// rounding and L's error (together below 7.3e-8 relative) move t by at most 7.3e-8 |y| log2 e,
+// so 2^t by at most 7.3e-8 |y| relative, about 1.2 |y| ulp, on top of the instruction's own
+// error; WGSL bounds exp by 3 + 2 |y| ulp, which linon_selftest_exp checks at every float in
+// [-87, 87].  Two instructions.  Large y gives +inf, small y 0 (or a subnormal), NaN gives NaN.
+__device__ __forceinline__ float exp_wgsl(float y) { return __builtin_amdgcn_exp2f(y * 1.44269502f); }
+// 1 / (1 + e^-x) at WGSL's accuracy: exp_wgsl and the hardware reciprocal (within 1 ulp, where
+// WGSL bounds the division by 2.5 ulp).  e^-x = +inf gives 0, NaN gives NaN.
+__device__ __forceinline__ float sigmoid_wgsl(float x) { return __builtin_amdgcn_rcpf(1.0f + exp_wgsl(-x)); }
 
 }  // namespace lw

