@@ -322,7 +322,7 @@ def main() -> int:
     d = per_field[dom]
     # the kernel each field runs under the default math choice, as rocprofv3 names it
     kname = {"swirl": "linon_wgslmath::raycast_kernel<lw::FieldSwirl>",
-             "lens": "linon::raycast_kernel<lw::FieldLens>",
+             "lens": "linon::raycast_kernel<lw::FieldLensT<true> >",
              "hot_air": "linon::raycast_kernel<lw::FieldHotAirT<true> >",
              "warp": "linon_wgslmath::raycast_kernel<lw::FieldWarp>"}[dom]
     roofline = {"bound": "valu", "kernel": f"void {kname}(RaycastParams)",

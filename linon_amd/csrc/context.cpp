@@ -246,7 +246,10 @@ void set_unbound_error(const std::string& msg) { g_unbound_error = msg; }
 const std::string& unbound_error() { return g_unbound_error; }
 
 int kernel_field_id(int math, int field) {
-    return (math == LINON_MATH_AUTO && field == LINON_FIELD_HOT_AIR) ? LINON_KERNEL_HOT_AIR_FAITHFUL_ROOT : field;
+    if (math != LINON_MATH_AUTO) return field;
+    if (field == LINON_FIELD_HOT_AIR) return LINON_KERNEL_HOT_AIR_FAITHFUL_ROOT;
+    if (field == LINON_FIELD_LENS) return LINON_KERNEL_LENS_RECIP;
+    return field;
 }
 
 bool wgsl_math_default(int math, int field) {

@@ -198,7 +198,8 @@ void fill_step(StepParams& I, const linon_integrator& ig, float field_weight);
 // The kernel build (WGSL accuracy or IEEE) that linon_math value `math` selects for a field.
 bool wgsl_math_default(int math, int field);
 // The kernel-table id a predefined field runs with under a math choice: under AUTO hot air takes
-// its faithful-root variant (LINON_KERNEL_HOT_AIR_FAITHFUL_ROOT), every other field its own id.
+// its faithful-root variant (LINON_KERNEL_HOT_AIR_FAITHFUL_ROOT) and the lens its variant with
+// the reciprocal above the floor (LINON_KERNEL_LENS_RECIP), every other field its own id.
 int kernel_field_id(int math, int field);
 // Error message of the calling thread's last failed entry point that takes no context
 // (linon_last_error(NULL) returns it).

@@ -87,7 +87,10 @@ struct FieldSwirl {  // rotateY(v0, 0.5 * length(p))
 #endif
     }
 };
-struct FieldLens {  // v - 0.3 * p / max(|p|^3, 1e-3)
+// Lens.  RecipAbove: the variant AUTO picks for lens, whose quotient above the floor is taken
+// at WGSL's accuracy (div3_lens_recip); |p|, the floor and its quotient stay exact.
+template <bool RecipAbove>
+struct FieldLensT {  // v - 0.3 * p / max(|p|^3, 1e-3)
     // |p| enters only through s = max(|p|^3, 0.001).  Where dot(p, p) < 2^-96 the root is below
     // 2^-47 whatever its last bits (or NaN, which max also drops), its cube below the floor, and
     // s the floor 0.001 either way; elsewhere sqrt_rn_no_tiny is sqrt_rn.  So |p| needs no
@@ -102,9 +105,12 @@ struct FieldLens {  // v - 0.3 * p / max(|p|^3, 1e-3)
         if (__all(__builtin_amdgcn_sqrtf(d2) < 0.0999f)) return v - div3_rn_lens_floor(0.3f * p);
 #endif
         const float r = sqrt_rn_no_tiny(d2);
-        return v - div3_rn_lens(0.3f * p, max(r * r * r, 0.001f));  // 0.3 p / max(r^3, 0.001)
+        const float s = max(r * r * r, 0.001f);
+        return v - (RecipAbove ? div3_lens_recip(0.3f * p, s) : div3_rn_lens(0.3f * p, s));  // 0.3 p / s
     }
 };
+using FieldLens = FieldLensT<false>;
+using FieldLensRecip = FieldLensT<true>;
 // Hot air: refraction(T(p_prev), T(p), v, normalize(p - c)), T = 20 + 80 sigmoid(8 (0.5 - |p - c|)).
 // FaithfulRoot: the variant AUTO picks for hot air.  Its temperature and normal are taken at
 // WGSL's accuracy: the distance |p - c| (the temperature's argument and the normal's length)

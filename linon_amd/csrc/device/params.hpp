@@ -13,6 +13,9 @@
 // kernel-table id (not a public field id) of hot air with the faithful distance root, which
 // the math choice AUTO runs hot air with (fields.hpp FieldHotAirT)
 #define LINON_KERNEL_HOT_AIR_FAITHFUL_ROOT 21
+// kernel-table id of the lens with its quotient above the floor at WGSL's accuracy, which the
+// math choice AUTO runs the lens with (fields.hpp FieldLensT)
+#define LINON_KERNEL_LENS_RECIP 22
 
 // counters[] slots
 #define LINON_CNT_STEPS 0

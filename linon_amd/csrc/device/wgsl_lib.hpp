@@ -224,6 +224,20 @@ __device__ __forceinline__ vec3 div3_rn_lens(vec3 v, float s) {
     return {v.x / s, v.y / s, v.z / s};
 #endif
 }
+// The lens's 0.3 p / s at WGSL's accuracy above the floor: the floor s = 0.001 keeps the exact
+// constant-divisor form (the captured centre, where rounding decides whether a ray stays held);
+// above it the three components multiply by the correctly rounded reciprocal of s, RN(v_i RN(1
+// / s)), within about one ulp of v_i / s where WGSL bounds the division by 2.5 ulp.  s is at
+// least 0.001 or NaN, so the reciprocal needs no range test.
+__device__ __forceinline__ vec3 div3_lens_recip(vec3 v, float s) {
+#ifdef LINON_FAST_MATH
+    return v / s;
+#else
+    if (s == 0.001f) return div3_rn_lens_floor(v);
+    const float y = recip_rn(s);
+    return {v.x * y, v.y * y, v.z * y};
+#endif
+}
 __device__ __forceinline__ vec3 normalize(vec3 a) { return div3_rn_bounded(a, length(a)); }
 __device__ __forceinline__ float distance(vec3 a, vec3 b) { return length(a - b); }
 __device__ __forceinline__ vec3 reflect(vec3 e1, vec3 e2) { return e1 - (2.0f * dot(e2, e1)) * e2; }

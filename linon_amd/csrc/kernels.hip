@@ -54,6 +54,8 @@ bool field_is_predefined(int id) { return (id >= 1 && id <= 9) || (id >= 16 && i
         case 20: return (const void*)&KERNEL<lw::FieldWarp>;           \
         case LINON_KERNEL_HOT_AIR_FAITHFUL_ROOT:                       \
             return (const void*)&KERNEL<lw::FieldHotAirFaithfulRoot>;  \
+        case LINON_KERNEL_LENS_RECIP:                                  \
+            return (const void*)&KERNEL<lw::FieldLensRecip>;           \
         default: return nullptr;                                       \
     }
 
@@ -83,6 +85,7 @@ const void* raycast_kernel_for(int id) {
         case 19: return (const void*)&raycast_kernel<lw::FieldHotAir>;
         case 20: return (const void*)&raycast_kernel<lw::FieldWarp>;
         case LINON_KERNEL_HOT_AIR_FAITHFUL_ROOT: return (const void*)&raycast_kernel<lw::FieldHotAirFaithfulRoot>;
+        case LINON_KERNEL_LENS_RECIP: return (const void*)&raycast_kernel<lw::FieldLensRecip>;
         default: return nullptr;
     }
 }

@@ -503,6 +503,8 @@ hipError_t sg_fill_i32(int* p, int n, int value, hipStream_t s) {
         case 20: return (const void*)&KERNEL<lw::FieldWarp>;                 \
         case LINON_KERNEL_HOT_AIR_FAITHFUL_ROOT:                             \
             return (const void*)&KERNEL<lw::FieldHotAirFaithfulRoot>;        \
+        case LINON_KERNEL_LENS_RECIP:                                        \
+            return (const void*)&KERNEL<lw::FieldLensRecip>;                 \
         case 3: return (const void*)&KERNEL<lw::FieldMirageSphericalSigmoid>; \
         case 16: return (const void*)&KERNEL<lw::FieldIdentity>;             \
         default: return nullptr;                                             \
