@@ -58,9 +58,10 @@ __device__ __forceinline__ vec3 ld_tri_vec(const float* tris, int face, int k) {
 }
 
 // WGSL-literal arithmetic: every operation rounded once, no FMA contraction, in the order the
-// shader text writes it.  The integrator, camera and sample-ray statements use it, so every
-// kernel that advances a ray (plain, batched, the surrogate's RK4 legs) rounds them alike
-// whatever the surrounding code lets the compiler fuse, and as the FP32 oracle does.
+// shader text writes it (and fma, where the integrator contracts on purpose: the RK4 stage
+// points).  The integrator, camera and sample-ray statements use it, so every kernel that
+// advances a ray (plain, batched, the surrogate's RK4 legs) rounds them alike whatever the
+// surrounding code lets the compiler fuse.
 namespace rn {
 __device__ __forceinline__ float mul(float a, float b) {
 #pragma clang fp contract(off)
@@ -78,6 +79,10 @@ __device__ __forceinline__ vec3 mul(float s, vec3 a) { return vec3f(mul(s, a.x),
 __device__ __forceinline__ vec3 mul(vec3 a, float s) { return vec3f(mul(a.x, s), mul(a.y, s), mul(a.z, s)); }
 __device__ __forceinline__ vec3 add(vec3 a, vec3 b) { return vec3f(add(a.x, b.x), add(a.y, b.y), add(a.z, b.z)); }
 __device__ __forceinline__ vec3 sub(vec3 a, vec3 b) { return vec3f(sub(a.x, b.x), sub(a.y, b.y), sub(a.z, b.z)); }
+// s a + b with one rounding per component
+__device__ __forceinline__ vec3 fma(float s, vec3 a, vec3 b) {
+    return vec3f(__builtin_fmaf(s, a.x, b.x), __builtin_fmaf(s, a.y, b.y), __builtin_fmaf(s, a.z, b.z));
+}
 __device__ __forceinline__ float dot(vec3 a, vec3 b) { return add(add(mul(a.x, b.x), mul(a.y, b.y)), mul(a.z, b.z)); }
 __device__ __forceinline__ vec3 cross(vec3 a, vec3 b) {
     return vec3f(sub(mul(a.y, b.z), mul(a.z, b.y)), sub(mul(a.z, b.x), mul(a.x, b.z)),
@@ -508,9 +513,12 @@ __device__ __forceinline__ bool ray_advance_impl(RayState& r, const StepParams& 
         // (position and direction; see opaque())
         const vec3 pf = opaque(p);
         vec3 k1 = rk4_stage1(field, pf, start_dir, opaque(r.d), r.t, 0);
-        vec3 k2 = field(pf, rn::add(p, rn::mul(hh, k1)), start_dir, k1, rn::add(r.t, hh));
-        vec3 k3 = field(pf, rn::add(p, rn::mul(hh, k2)), start_dir, k2, rn::add(r.t, hh));
-        vec3 k4 = field(pf, rn::add(p, rn::mul(h, k3)), start_dir, k3, rn::add(r.t, h));
+        // stage points p + a k as one FMA per component: a contraction a WGSL driver may make of
+        // src/main_view.wgsl's p + h / 2 * k (as the oracle's FMA-contracted build lets its
+        // compiler do), written out so that every kernel advancing a ray rounds it the same way
+        vec3 k2 = field(pf, rn::fma(hh, k1, p), start_dir, k1, rn::add(r.t, hh));
+        vec3 k3 = field(pf, rn::fma(hh, k2, p), start_dir, k2, rn::add(r.t, hh));
+        vec3 k4 = field(pf, rn::fma(h, k3, p), start_dir, k3, rn::add(r.t, h));
         vec3 V = rk4_average(k1, k2, k3, k4);
         float diff = 0.0f;
         if (adaptive) {  // src/main_view.wgsl:251-255
