@@ -258,7 +258,10 @@ linon_status linon_read_grid(linon_context* ctx, int32_t* dims, uint32_t* cell_s
  * refraction fields (the four mirages, hot air), where the eta*cos - sqrt(k) cancellation near
  * grazing incidence amplifies division rounding, for rotation (hardware sin/cos of pi*t up to
  * 5 pi moves hit faces), for lens (rays captured near the r^3 singularity end elsewhere), and
- * for WGSL bodies.  (No counterpart in the reference; its driver decides.) */
+ * for WGSL bodies.  Under AUTO, hot air takes its temperature and normal at WGSL accuracy with
+ * refraction correctly rounded, and lens its quotient above the singularity's floor at WGSL
+ * accuracy with |p| and the floor exact; IEEE runs both fields exactly.  (No counterpart in
+ * the reference; its driver decides.) */
 typedef enum linon_math {
     LINON_MATH_AUTO = 0,
     LINON_MATH_IEEE = 1,
