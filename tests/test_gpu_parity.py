@@ -99,6 +99,20 @@ def test_benchmark_scene_small(view, field):
     _case(view, 48, 32, wl.camera(), wl.settings(), wl.integrator(), field, v, f, f"bench-scene-{field}")
 
 
+@pytest.mark.parametrize("field", ["lens", "hot_air"])
+def test_benchmark_scene_small_exact_build(view, field):
+    """The exact functors MATH_IEEE runs for lens and hot air (every quotient, root and
+    exponential correctly rounded; the default math choice runs variants) against the oracle
+    on the benchmark scene."""
+    wl = configs.OVERLAY_1080P
+    v, f = scenes.benchmark_scene(level=2)
+    view.set_math(L.MATH_IEEE)
+    try:
+        _case(view, 48, 32, wl.camera(), wl.settings(), wl.integrator(), field, v, f, f"bench-scene-ieee-{field}")
+    finally:
+        view.set_math(L.MATH_AUTO)
+
+
 @pytest.mark.parametrize("clip_exit", [0, 1])
 @pytest.mark.parametrize("need_mapping", [0, 1])
 @pytest.mark.parametrize("field", ["swirl", "lens", "hot_air", "warp"])
