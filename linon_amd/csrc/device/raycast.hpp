@@ -19,8 +19,9 @@
 
 // Waves per SIMD the ray-cast kernels (static and run-time compiled) are built for.  6 caps a
 // kernel at 80 VGPRs; the compiler's own choice is 94-103 (4-5 waves).  At 6 a few values
-// spill outside the step loop, and the four 1080p benchmark fields render 5-12% faster than at
-// 4-5 waves and faster than at 7 (2-7%) or 8.  0 leaves the choice to the compiler.
+// spill outside the step loop; the four 1080p benchmark fields render faster than at 4 waves,
+// as fast as at 5 (within run-to-run noise) and faster than at 7 (2-7%) or 8.  0 leaves the
+// choice to the compiler.
 #ifndef LK_RAYCAST_WAVES_PER_EU
 #define LK_RAYCAST_WAVES_PER_EU 6
 #endif
