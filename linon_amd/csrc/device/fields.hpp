@@ -120,7 +120,7 @@ using FieldLensRecip = FieldLensT<true>;
 // sqrt(k), where rounding turns into exit error near grazing incidence) stays correctly
 // rounded.  Its exit points pass every parity test as the exact variant's do, in about 40%
 // fewer instructions per step.  The temperature enters the step only through the refraction
-// indices, whose relative slope is at most 3.4e-3 per degree: an ulp of the sigmoid moves them
+// indices, whose relative slope is at most 3.5e-3 per degree: an ulp of the sigmoid moves them
 // by about 1e-8, below the index ratio's own rounding.
 template <bool FaithfulRoot>
 struct FieldHotAirT {

@@ -119,3 +119,29 @@ def test_rk4_division_by_six():
             assert (np.abs(want[bad].astype(np.float64) - got[bad].astype(np.float64)) == 2.0 ** -149).all()
             ties += int(bad.sum())
     assert ties > 0
+
+
+def test_hot_air_temperature_accuracy_does_not_reach_the_index_ratio():
+    """Hot air's default variant takes its sigmoid at WGSL's accuracy (the hardware 2^t of
+    RN(y log2 e) and the hardware reciprocal).  The temperature T = 20 + 80 s enters the step only
+    through the refraction indices n(T) = c1 P (1 + P (60.1 - 0.972 T) 1e-10) / (1 + 0.00366 T)
+    (src/main_view.wgsl:111-146).  Checked here: RN(log2 e) and the product's rounding err by at
+    most 7.3e-8 relative; that error moves T by at most 80 max(y e^-y) 7.3e-8 <= 2.2e-6 degrees
+    and an ulp of s by 80 2^-24; n's relative slope over hot air's [20, 100] degrees is at most
+    3.5e-3 per degree; so either moves n(T) by less than 2e-8 relative, below a half ulp of
+    the index ratio (2^-24 relative at least 5.9e-8)."""
+    L = F(int(np.float32(np.log2(np.e)).view(np.uint32)) & 0x7FFFFF | 0x800000, 2 ** 23)
+    rel_L = abs(float(L) - np.log2(np.e)) / np.log2(np.e)
+    rel = rel_L + 2.0 ** -24
+    assert rel <= 7.3e-8, rel
+    y = np.linspace(0.0, 100.0, 200001)
+    dT_exp = 80.0 * np.max(y * np.exp(-y)) * 7.3e-8
+    assert dT_exp <= 2.2e-6, dT_exp
+    dT_ulp = 80.0 * 2.0 ** -24
+    P, c1 = 101325.0, 0.0000104
+    T = np.linspace(20.0, 100.0, 80001)
+    n = c1 * P * (1.0 + P * (60.1 - 0.972 * T) * 1e-10) / (1.0 + 0.00366 * T)
+    slope = np.max(np.abs(np.gradient(np.log(n), T)))
+    assert slope <= 3.5e-3, slope
+    moved = slope * max(dT_exp, dT_ulp)
+    assert moved < 2e-8 < 2.0 ** -24, moved
