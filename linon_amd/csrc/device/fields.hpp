@@ -162,7 +162,15 @@ using FieldHotAir = FieldHotAirT<false>;
 using FieldHotAirFaithfulRoot = FieldHotAirT<true>;
 struct FieldWarp {  // v0 + 0.3 * (sin 4p.y, 0, sin 4p.x)
     __device__ __forceinline__ vec3 operator()(vec3, vec3 p, vec3 v0, vec3, float) const {
+#ifdef LINON_FAST_MATH
+        // the hardware sine takes turns: sin(4 x) is v_sin(RN(RN(4 x) RN(1 / (2 pi)))), and
+        // RN(4 x) = 4 x exactly (below overflow), so one product with 4 RN(1 / (2 pi)) gives the
+        // same argument, one multiplication fewer per sine; frames bit-identical
+        return v0 + 0.3f * vec3f(__builtin_amdgcn_sinf(p.y * (4.0f * INV_2PI)), 0.0f,
+                                 __builtin_amdgcn_sinf(p.x * (4.0f * INV_2PI)));
+#else
         return v0 + 0.3f * vec3f(sin(4.0f * p.y), 0.0f, sin(4.0f * p.x));
+#endif
     }
 };
 
